@@ -1,0 +1,1214 @@
+// Host side of the MI355X grid backend: the C-ABI of src/grid/grid_task_list.h,
+// src/grid/common/grid_library.h, src/grid/common/grid_basis_set.h and
+// src/offload/offload_buffer.h, plus the backend-level grid_gpu_* symbols and
+// the extensions in include/grid_mi355x.h.
+//
+// Compiled with -ffp-contract=off: the task geometry computed here (cube
+// centres, discretised radii, index boxes) must round exactly like the
+// reference's, because it decides which grid points belong to a task.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "../../include/grid_library.h"
+#include "../../include/grid_mi355x.h"
+#include "../../include/grid_task_list.h"
+#include "grid_internal.h"
+
+namespace gm {
+hipError_t launch_collocate(const PointParams &P, int nwaves, hipStream_t s);
+hipError_t launch_integrate(const PointParams &P, int max_lp, int nwaves, hipStream_t s);
+hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,
+                             long long pab_len, const double *sphi, double *cart,
+                             long long cart_len, int cach, int max_nsb, hipStream_t s);
+hipError_t launch_contract(const Group *groups, int ngroups, const double *hab_cart,
+                           const double *sphi, double *hab, long long hab_len, int cach,
+                           int max_nsb, int max_nsa_nsb, hipStream_t s);
+hipError_t launch_fill(double *p, long long n, double v, hipStream_t s);
+hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *task_out,
+                                double *forces, int natoms, hipStream_t s);
+hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
+                                hipStream_t s);
+}  // namespace gm
+
+using namespace gm;
+
+// ---------------------------------------------------------------------------
+// Errors: the reference has no return codes and aborts (src/grid/grid_task_list.c
+// :124-133, src/offload/offload_runtime.h:66-79); so does this library.
+// ---------------------------------------------------------------------------
+[[noreturn]] static void fatal(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  fprintf(stderr, "grid_mi355x error: ");
+  vfprintf(stderr, fmt, ap);
+  fprintf(stderr, "\n");
+  va_end(ap);
+  fflush(stderr);
+  abort();
+}
+
+#define HIP_CHECK(call)                                                        \
+  do {                                                                         \
+    const hipError_t err_ = (call);                                            \
+    if (err_ != hipSuccess)                                                    \
+      fatal("%s failed at %s:%d: %s", #call, __FILE__, __LINE__,               \
+            hipGetErrorString(err_));                                          \
+  } while (0)
+
+static inline int ncoset(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }
+static inline int pmod(int a, int m) { return ((a % m) + m) % m; }
+
+// ---------------------------------------------------------------------------
+// Library state (src/grid/common/grid_library.c)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNumKernelIds = 6;
+constexpr int kCounterLp = 20;
+
+struct PendingTiming {
+  int id;
+  hipEvent_t a, b;
+};
+
+struct LibState {
+  std::mutex mu;
+  bool initialized = false;
+  grid_library_config config = {GRID_BACKEND_AUTO, false, false};
+  hipStream_t own_stream = nullptr;
+  hipStream_t user_stream = nullptr;
+  bool stream_set = false;
+  int precision = -1;
+  bool timing = false;
+  double kernel_ms[kNumKernelIds] = {};
+  long kernel_n[kNumKernelIds] = {};
+  std::vector<PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  long counters[4][kCounterLp] = {};
+} S;
+
+hipStream_t stream() {
+  if (S.stream_set) return S.user_stream;
+  if (!S.own_stream) HIP_CHECK(hipStreamCreateWithFlags(&S.own_stream, hipStreamNonBlocking));
+  return S.own_stream;
+}
+
+int precision() {
+  if (S.precision < 0) {
+    const char *e = getenv("GRID_MI355X_PRECISION");
+    S.precision = (e && atoi(e) == 1) ? GRID_MI355X_FP32_MIXED : GRID_MI355X_FP64;
+  }
+  return S.precision;
+}
+
+hipEvent_t get_event() {
+  if (!S.event_pool.empty()) {
+    hipEvent_t e = S.event_pool.back();
+    S.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+// Brackets one launch with events when timing is enabled.
+struct TimedLaunch {
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipStream_t s;
+  TimedLaunch(int id_, hipStream_t s_) : id(id_), s(s_) {
+    if (S.timing) {
+      a = get_event();
+      b = get_event();
+      HIP_CHECK(hipEventRecord(a, s));
+    }
+  }
+  ~TimedLaunch() {
+    if (S.timing) {
+      (void)hipEventRecord(b, s);
+      S.pending.push_back({id, a, b});
+    }
+  }
+};
+
+void resolve_timings() {
+  for (auto &p : S.pending) {
+    HIP_CHECK(hipEventSynchronize(p.b));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    S.kernel_ms[p.id] += ms;
+    S.kernel_n[p.id] += 1;
+    S.event_pool.push_back(p.a);
+    S.event_pool.push_back(p.b);
+  }
+  S.pending.clear();
+}
+
+void sync_stream(hipStream_t s) {
+  HIP_CHECK(hipStreamSynchronize(s));
+  resolve_timings();
+}
+
+void ensure_device() {
+  int n = 0;
+  const hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    fatal("no HIP device is visible (%s); the MI355X grid backend has no CPU path",
+          e != hipSuccess ? hipGetErrorString(e) : "device count 0");
+}
+
+// Device allocation that grows on demand.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void *get(size_t n) {
+    if (n > bytes) {
+      if (p) HIP_CHECK(hipFree(p));
+      p = nullptr;
+      bytes = 0;
+      if (n) HIP_CHECK(hipMalloc(&p, n));
+      bytes = n;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// l-range shifts of cab for a collocate func, src/grid/common/grid_prepare_pab.h:447-517.
+bool func_ldiffs(int func, int d[4]) {
+  int a, b;
+  if (func == 100) {
+    a = b = 0;
+  } else if (func == 200 || (func >= 301 && func <= 303) || (func >= 501 && func <= 503) ||
+             (func >= 601 && func <= 603) || (func >= 901 && func <= 906) ||
+             (func >= 1001 && func <= 1003)) {
+    a = b = 1;
+  } else if (func >= 411 && func <= 433 && func % 10 >= 1 && func % 10 <= 3 &&
+             (func / 10) % 10 >= 1 && (func / 10) % 10 <= 3) {
+    d[0] = 1; d[1] = -1; d[2] = 2; d[3] = -1;
+    return true;
+  } else if ((func >= 701 && func <= 703) || (func >= 801 && func <= 803)) {
+    a = b = 2;
+  } else {
+    return false;
+  }
+  d[0] = a; d[1] = -a; d[2] = b; d[3] = -b;
+  return true;
+}
+
+// process_get_ldiffs, src/grid/common/grid_process_vab.h:222-251.
+void integrate_ldiffs(bool forces, bool virial, bool tau, int d[4]) {
+  d[0] = d[1] = d[2] = d[3] = 0;
+  if (forces || virial) { d[0] += 1; d[1] -= 1; d[3] -= 1; }
+  if (virial) { d[0] += 1; d[2] += 1; }
+  if (tau) { d[0] += 1; d[2] += 1; d[1] -= 1; d[3] -= 1; }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Task list
+// ---------------------------------------------------------------------------
+namespace {
+struct TaskListImpl {
+  static constexpr unsigned kMagic = 0x6d693335u;
+  unsigned magic = kMagic;
+  bool empty = true;
+  bool orthorhombic = false;
+  int ntasks = 0, nlevels = 0, natoms = 0, nkinds = 0, nblocks = 0, ngroups = 0;
+  std::vector<Layout> layouts;
+  std::vector<int> npts_local;  // [nlevels][3]
+  // device-resident description
+  DevBuf d_tasks, d_groups, d_sphi, d_fptr, d_fent, d_levels, d_cart, d_hab_cart, d_task_out;
+  Level *h_levels = nullptr;  // pinned staging for the per-call level table
+  long long cart_len = 0;
+  // maxima for the LDS plan
+  int max_la = 0, max_lb = 0, max_lp = 0, max_cmax = 0, max_rows_ortho = 0;
+  int max_range = 0, max_rows_general = 0, max_planes = 0;
+  bool has_ortho = false, has_general = false;
+  int max_ncoa = 0, max_nsb = 0, max_nsa_nsb = 0;
+  long long block_end = 0;  // max over groups of block offset + block size
+  long n_ortho = 0, n_general = 0, n_skip = 0;
+  long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
+  long long points = -1;
+  // per-call scratch when the caller's buffers have no device mirror
+  DevBuf pab_dev, hab_dev, forces_dev;
+  std::vector<DevBuf> grid_dev;
+
+  ~TaskListImpl() {
+    DevBuf *bufs[] = {&d_tasks, &d_groups, &d_sphi, &d_fptr, &d_fent, &d_levels,
+                      &d_cart, &d_hab_cart, &d_task_out, &pab_dev, &hab_dev, &forces_dev};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &b : grid_dev) b.release();
+    if (h_levels) (void)hipHostFree(h_levels);
+    magic = 0;
+  }
+};
+
+TaskListImpl *checked(const void *p) {
+  TaskListImpl *t = (TaskListImpl *)p;
+  if (!t || t->magic != TaskListImpl::kMagic) fatal("invalid or freed task list handle");
+  return t;
+}
+
+template <typename T>
+void upload(DevBuf &b, const std::vector<T> &v) {
+  const size_t n = v.size() * sizeof(T);
+  void *p = b.get(n ? n : sizeof(T));
+  if (n) HIP_CHECK(hipMemcpy(p, v.data(), n, hipMemcpyHostToDevice));
+}
+
+void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevels,
+                     int natoms, int nkinds, int nblocks, const int *block_offsets,
+                     const double *atom_positions, const int *atom_kinds,
+                     const grid_basis_set *const *basis_sets, const int *level_list,
+                     const int *iatom_list, const int *jatom_list, const int *iset_list,
+                     const int *jset_list, const int *ipgf_list, const int *jpgf_list,
+                     const int *border_mask_list, const int *block_num_list,
+                     const double *radius_list, const double *rab_list,
+                     const int *npts_global, const int *npts_local, const int *shift_local,
+                     const int *border_width, const double *dh, const double *dh_inv) {
+  const grid_library_config cfg = grid_library_get_config();
+  if (cfg.backend != GRID_BACKEND_AUTO && cfg.backend != GRID_BACKEND_GPU)
+    fatal("grid backend %d is not provided by this library (only AUTO/GPU, the "
+          "MI355X backend)", (int)cfg.backend);
+  if (cfg.validate)
+    fatal("VALIDATE needs CP2K's ref backend, which this library does not contain");
+
+  tl->orthorhombic = orthorhombic;
+  tl->ntasks = ntasks;
+  tl->nlevels = nlevels;
+  tl->natoms = natoms;
+  tl->nkinds = nkinds;
+  tl->nblocks = nblocks;
+  tl->npts_local.assign(npts_local, npts_local + 3 * (size_t)std::max(nlevels, 0));
+  // src/grid/grid_task_list.c:65-70
+  tl->empty = (nblocks == 0 || ntasks == 0 || nlevels == 0);
+  if (tl->empty) return;
+  ensure_device();
+
+  tl->layouts.resize(nlevels);
+  for (int l = 0; l < nlevels; l++) {
+    Layout &y = tl->layouts[l];
+    for (int i = 0; i < 3; i++) {
+      y.npts_global[i] = npts_global[3 * l + i];
+      y.npts_local[i] = npts_local[3 * l + i];
+      y.shift_local[i] = shift_local[3 * l + i];
+      y.border_width[i] = border_width[3 * l + i];
+      if (y.npts_local[i] <= 0 || y.npts_global[i] <= 0)
+        fatal("level %d: non-positive grid dimension", l + 1);
+      for (int j = 0; j < 3; j++) {
+        y.dh[i][j] = dh[9 * l + 3 * i + j];
+        y.dh_inv[i][j] = dh_inv[9 * l + 3 * i + j];
+      }
+    }
+  }
+
+  // --- validate indices (the reference trusts its caller; a wrong index here
+  //     would become an out-of-bounds access on the device) ---
+  int max_set_l = 0;
+  for (int k = 0; k < nkinds; k++) {
+    const grid_basis_set *b = basis_sets[k];
+    if (!b) fatal("basis set of kind %d is NULL", k + 1);
+    for (int s = 0; s < b->nset; s++) {
+      if (b->lmax[s] > 5 || b->lmin[s] < 0 || b->lmin[s] > b->lmax[s])
+        fatal("kind %d set %d: unsupported l range %d..%d (max 5)", k + 1, s + 1,
+              b->lmin[s], b->lmax[s]);
+      if (b->npgf[s] * ncoset(b->lmax[s]) > b->maxco)
+        fatal("kind %d set %d: npgf*ncoset(lmax) exceeds maxco", k + 1, s + 1);
+      if (b->first_sgf[s] < 1 || b->first_sgf[s] - 1 + b->nsgf_set[s] > b->nsgf)
+        fatal("kind %d set %d: spherical functions outside 1..nsgf", k + 1, s + 1);
+      max_set_l = std::max(max_set_l, b->lmax[s]);
+    }
+  }
+  for (int a = 0; a < natoms; a++)
+    if (atom_kinds[a] < 1 || atom_kinds[a] > nkinds) fatal("atom %d: bad kind", a + 1);
+  for (int t = 0; t < ntasks; t++) {
+    const int lv = level_list[t], ia = iatom_list[t], ja = jatom_list[t];
+    if (lv < 1 || lv > nlevels) fatal("task %d: level %d out of range", t + 1, lv);
+    if (ia < 1 || ia > natoms || ja < 1 || ja > natoms) fatal("task %d: atom out of range", t + 1);
+    if (block_num_list[t] < 1 || block_num_list[t] > nblocks)
+      fatal("task %d: block %d out of range", t + 1, block_num_list[t]);
+    const grid_basis_set *ib = basis_sets[atom_kinds[ia - 1] - 1];
+    const grid_basis_set *jb = basis_sets[atom_kinds[ja - 1] - 1];
+    if (iset_list[t] < 1 || iset_list[t] > ib->nset || jset_list[t] < 1 || jset_list[t] > jb->nset)
+      fatal("task %d: set out of range", t + 1);
+    if (ipgf_list[t] < 1 || ipgf_list[t] > ib->npgf[iset_list[t] - 1] || jpgf_list[t] < 1 ||
+        jpgf_list[t] > jb->npgf[jset_list[t] - 1])
+      fatal("task %d: pgf out of range", t + 1);
+  }
+
+  // --- order: level, block, iset, jset (src/grid/ref/grid_ref_task_list.c:26-37) ---
+  std::vector<int> order(ntasks);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (level_list[a] != level_list[b]) return level_list[a] < level_list[b];
+    if (block_num_list[a] != block_num_list[b]) return block_num_list[a] < block_num_list[b];
+    if (iset_list[a] != iset_list[b]) return iset_list[a] < iset_list[b];
+    return jset_list[a] < jset_list[b];
+  });
+
+  // --- groups: one per (block, iset, jset) over all levels ---
+  std::vector<long long> keys(ntasks);
+  long long smax = 1;
+  for (int k = 0; k < nkinds; k++) smax = std::max<long long>(smax, basis_sets[k]->nset);
+  for (int t = 0; t < ntasks; t++)
+    keys[t] = ((long long)(block_num_list[t] - 1) * smax + (iset_list[t] - 1)) * smax +
+              (jset_list[t] - 1);
+  std::vector<long long> ukeys(keys);
+  std::sort(ukeys.begin(), ukeys.end());
+  ukeys.erase(std::unique(ukeys.begin(), ukeys.end()), ukeys.end());
+  tl->ngroups = (int)ukeys.size();
+  std::vector<Group> groups(tl->ngroups);
+  std::vector<int> group_of(ntasks);
+  std::vector<char> group_init(tl->ngroups, 0);
+  // sphi of all kinds, concatenated
+  std::vector<long long> sphi_off(nkinds);
+  std::vector<double> sphi_all;
+  for (int k = 0; k < nkinds; k++) {
+    const grid_basis_set *b = basis_sets[k];
+    sphi_off[k] = (long long)sphi_all.size();
+    sphi_all.insert(sphi_all.end(), b->sphi, b->sphi + (size_t)b->nsgf * b->maxco);
+  }
+  tl->max_ncoa = tl->max_nsb = tl->max_nsa_nsb = 0;
+  tl->block_end = 0;
+  for (int t = 0; t < ntasks; t++) {
+    const int g = (int)(std::lower_bound(ukeys.begin(), ukeys.end(), keys[t]) - ukeys.begin());
+    group_of[t] = g;
+    const int ia = iatom_list[t] - 1, ja = jatom_list[t] - 1;
+    if (group_init[g]) {
+      if (groups[g].iatom != ia || groups[g].jatom != ja)
+        fatal("block %d is used with two different atom pairs", block_num_list[t]);
+      continue;
+    }
+    group_init[g] = 1;
+    const int ik = atom_kinds[ia] - 1, jk = atom_kinds[ja] - 1;
+    const grid_basis_set *ib = basis_sets[ik], *jb = basis_sets[jk];
+    const int is = iset_list[t] - 1, js = jset_list[t] - 1;
+    Group &G = groups[g];
+    G.iatom = ia;
+    G.jatom = ja;
+    G.block_off = block_offsets[block_num_list[t] - 1];
+    G.transpose = ia <= ja;
+    G.ncoa = ib->npgf[is] * ncoset(ib->lmax[is]);
+    G.ncob = jb->npgf[js] * ncoset(jb->lmax[js]);
+    G.nsa = ib->nsgf_set[is];
+    G.nsb = jb->nsgf_set[js];
+    G.sa0 = ib->first_sgf[is] - 1;
+    G.sb0 = jb->first_sgf[js] - 1;
+    G.nsgfa = ib->nsgf;
+    G.nsgfb = jb->nsgf;
+    G.maxcoa = ib->maxco;
+    G.maxcob = jb->maxco;
+    G.sphi_a = sphi_off[ik] + (long long)G.sa0 * ib->maxco;
+    G.sphi_b = sphi_off[jk] + (long long)G.sb0 * jb->maxco;
+    if (G.block_off < 0) fatal("block %d has a negative offset", block_num_list[t]);
+    tl->block_end = std::max(tl->block_end, (long long)G.block_off + (long long)G.nsgfa * G.nsgfb);
+    tl->max_ncoa = std::max(tl->max_ncoa, G.ncoa);
+    tl->max_nsb = std::max(tl->max_nsb, G.nsb);
+    tl->max_nsa_nsb = std::max(tl->max_nsa_nsb, G.nsa * G.nsb);
+  }
+  long long cart = 0;
+  for (auto &G : groups) {
+    G.cart_off = cart;
+    cart += (long long)G.ncoa * G.ncob;
+  }
+  tl->cart_len = cart;
+
+  // --- per-task geometry (FP64, operation order of the reference) ---
+  std::vector<Task> tasks(ntasks);
+  tl->max_la = tl->max_lb = tl->max_lp = tl->max_cmax = tl->max_rows_ortho = 0;
+  tl->max_range = tl->max_rows_general = tl->max_planes = 0;
+  tl->has_ortho = tl->has_general = false;
+  tl->n_ortho = tl->n_general = tl->n_skip = 0;
+  memset(tl->hist, 0, sizeof(tl->hist));
+  for (int s = 0; s < ntasks; s++) {
+    const int t = order[s];
+    Task &T = tasks[s];
+    memset(&T, 0, sizeof(T));
+    const int ia = iatom_list[t] - 1, ja = jatom_list[t] - 1;
+    const grid_basis_set *ib = basis_sets[atom_kinds[ia] - 1];
+    const grid_basis_set *jb = basis_sets[atom_kinds[ja] - 1];
+    const int is = iset_list[t] - 1, js = jset_list[t] - 1;
+    const int ip = ipgf_list[t] - 1, jp = jpgf_list[t] - 1;
+    const int level = level_list[t] - 1;
+    const Layout &y = tl->layouts[level];
+    const int bm = border_mask_list[t];
+    T.level = level;
+    T.group = group_of[t];
+    T.iatom = ia;
+    T.jatom = ja;
+    T.la_max = ib->lmax[is];
+    T.la_min = ib->lmin[is];
+    T.lb_max = jb->lmax[js];
+    T.lb_min = jb->lmin[js];
+    T.o1 = ip * ncoset(T.la_max);
+    T.o2 = jp * ncoset(T.lb_max);
+    T.zeta = ib->zet[is * ib->maxpgf + ip];
+    T.zetb = jb->zet[js * jb->maxpgf + jp];
+    T.rscale = (ia == ja) ? 1.0 : 2.0;
+    T.radius = radius_list[t];
+    for (int d = 0; d < 3; d++) {
+      T.ra[d] = atom_positions[3 * ia + d];
+      T.rab[d] = rab_list[3 * t + d];
+    }
+    // cab_to_grid, src/grid/ref/grid_ref_collint.h:928-947
+    double hmax = 0.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) hmax = fmax(hmax, fabs(y.dh[i][j]));
+    T.zetp = T.zeta + T.zetb;
+    const double f = T.zetb / T.zetp;
+    const double rab2 = T.rab[0] * T.rab[0] + T.rab[1] * T.rab[1] + T.rab[2] * T.rab[2];
+    T.pexp = exp(-T.zeta * f * rab2);
+    for (int d = 0; d < 3; d++) {
+      T.rp[d] = T.ra[d] + f * T.rab[d];
+      T.rb[d] = T.ra[d] + T.rab[d];
+    }
+    const int lp0 = T.la_max + T.lb_max;
+    tl->max_la = std::max(tl->max_la, T.la_max);
+    tl->max_lb = std::max(tl->max_lb, T.lb_max);
+    tl->max_lp = std::max(tl->max_lp, lp0);
+    if (2.0 * T.radius < hmax) {
+      T.flags = kTaskSkip;
+      tl->n_skip++;
+      continue;
+    }
+    if (orthorhombic && bm == 0) {
+      // ortho_cxyz_to_grid, grid_ref_collint.h:222-257
+      T.flags = kTaskOrtho;
+      for (int d = 0; d < 3; d++) {
+        double s2 = 0.0;
+        for (int e = 0; e < 3; e++) s2 += y.dh_inv[e][d] * T.rp[e];
+        T.center[d] = (int)floor(s2);
+      }
+      for (int d = 0; d < 3; d++) T.roff[d] = T.rp[d] - ((double)T.center[d]) * y.dh[d][d];
+      const double drmin = fmin(y.dh[0][0], fmin(y.dh[1][1], y.dh[2][2]));
+      T.disr = drmin * fmax(1.0, ceil(T.radius / drmin));
+      T.cmax = 0;
+      for (int d = 0; d < 3; d++) {
+        T.lo[d] = (int)ceil(-1e-8 - T.disr * y.dh_inv[d][d]);
+        T.hi[d] = 1 - T.lo[d];
+        if (y.npts_global[d] != y.npts_local[d]) {
+          // non-periodic direction: the cube must fit without wrapping (:247-253)
+          const int off = pmod(T.center[d] + T.lo[d] - y.shift_local[d], y.npts_global[d]) - T.lo[d];
+          if (!(off + T.hi[d] < y.npts_local[d]) || !(off + T.lo[d] >= 0))
+            fatal("task %d: cube does not fit into the non-periodic local grid", t + 1);
+        }
+        T.cmax = std::max(T.cmax, T.hi[d]);
+      }
+      tl->has_ortho = true;
+      tl->max_cmax = std::max(tl->max_cmax, T.cmax);
+      tl->max_rows_ortho = std::max(tl->max_rows_ortho, 2 - 2 * T.lo[1]);
+      tl->max_planes = std::max(tl->max_planes, 2 - 2 * T.lo[2]);
+      tl->n_ortho++;
+      tl->hist[1][std::min(lp0, kCounterLp - 1)]++;
+    } else {
+      // general_cijk_to_grid, grid_ref_collint.h:591-654
+      for (int d = 0; d < 3; d++) {
+        T.blo[d] = 0;
+        T.bhi[d] = y.npts_local[d] - 1;
+        if (bm & (1 << (2 * d))) T.blo[d] += y.border_width[d];
+        if (bm & (1 << (2 * d + 1))) T.bhi[d] -= y.border_width[d];
+        T.gp[d] = 0.0;
+        for (int e = 0; e < 3; e++) T.gp[d] += y.dh_inv[e][d] * T.rp[e];
+        T.lo[d] = 2147483647;
+        T.hi[d] = -2147483647 - 1;
+      }
+      for (int si = -1; si <= 1; si++)
+        for (int sj = -1; sj <= 1; sj++)
+          for (int sk = -1; sk <= 1; sk++) {
+            const double x = T.rp[0] + si * T.radius;
+            const double yy = T.rp[1] + sj * T.radius;
+            const double z = T.rp[2] + sk * T.radius;
+            for (int d = 0; d < 3; d++) {
+              const double r = y.dh_inv[0][d] * x + y.dh_inv[1][d] * yy + y.dh_inv[2][d] * z;
+              T.lo[d] = std::min(T.lo[d], (int)floor(r));
+              T.hi[d] = std::max(T.hi[d], (int)ceil(r));
+            }
+          }
+      tl->has_general = true;
+      for (int d = 0; d < 3; d++) tl->max_range = std::max(tl->max_range, T.hi[d] - T.lo[d] + 1);
+      tl->max_rows_general = std::max(tl->max_rows_general, T.hi[1] - T.lo[1] + 1);
+      tl->max_planes = std::max(tl->max_planes, T.hi[2] - T.lo[2] + 1);
+      tl->n_general++;
+      tl->hist[0][std::min(lp0, kCounterLp - 1)]++;
+    }
+  }
+  (void)max_set_l;
+
+  // --- force CSR: atom -> (2*task + side) ---
+  std::vector<int> fptr(natoms + 1, 0), fent(2 * (size_t)ntasks);
+  for (int s = 0; s < ntasks; s++) {
+    fptr[tasks[s].iatom + 1]++;
+    fptr[tasks[s].jatom + 1]++;
+  }
+  for (int a = 0; a < natoms; a++) fptr[a + 1] += fptr[a];
+  std::vector<int> fill(fptr.begin(), fptr.end() - 1);
+  for (int s = 0; s < ntasks; s++) {
+    fent[fill[tasks[s].iatom]++] = 2 * s;
+    fent[fill[tasks[s].jatom]++] = 2 * s + 1;
+  }
+
+  upload(tl->d_tasks, tasks);
+  upload(tl->d_groups, groups);
+  upload(tl->d_sphi, sphi_all);
+  upload(tl->d_fptr, fptr);
+  upload(tl->d_fent, fent);
+  tl->d_levels.get(sizeof(Level) * nlevels);
+  if (!tl->h_levels) HIP_CHECK(hipHostMalloc((void **)&tl->h_levels, sizeof(Level) * 64));
+  if (nlevels > 64) fatal("at most 64 grid levels are supported");
+  tl->d_cart.get(sizeof(double) * std::max<long long>(tl->cart_len, 1));
+  tl->d_hab_cart.get(sizeof(double) * std::max<long long>(tl->cart_len, 1));
+  tl->d_task_out.get(sizeof(double) * kTaskOutStride * (size_t)ntasks);
+  tl->grid_dev.resize(nlevels);
+  tl->points = -1;
+}
+
+// LDS plan for one launch; ld are the l-range shifts of cab.
+LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
+  LdsPlan p{};
+  const int la = tl->max_la + ld[0], lb = tl->max_lb + ld[2];
+  const int lp = std::min(tl->max_lp + ld[0] + ld[2], la + lb);
+  p.np = lp + 1;
+  p.ext = 2 * tl->max_cmax + 2;
+  p.mapn = std::max(2 * tl->max_cmax + 1, tl->max_range);
+  const int need_rows = std::max({tl->max_rows_ortho, tl->max_rows_general, 1});
+  const int n_cab = ncoset(la) * ncoset(lb);
+  const int n_alpha = 3 * (la + 1) * (lb + 1) * p.np;
+  const int n_cube = p.np * p.np * p.np;
+  auto layout = [&](int rowcap, int plcap) {
+    p.rowcap = rowcap;
+    p.plcap = plcap;
+    int o = 0;
+    p.o_cab = o; o += n_cab;
+    p.o_alpha = o; o += n_alpha;
+    p.o_cxyz = o; o += n_cube;
+    p.o_cijk = o; o += tl->has_general ? n_cube : 0;
+    p.o_tab = o; o += tl->has_ortho ? 3 * p.np * p.ext : 0;
+    p.o_cxy = o; o += plcap * p.np * p.np;
+    p.o_cxrow = o; o += rowcap * p.np;
+    p.o_red = o; o += 4 * kTaskOutStride;
+    p.n_doubles = o;
+    int i = 0;
+    p.i_map = i; i += 3 * p.mapn;
+    p.i_plane = i; i += 3 * (plcap + 1);
+    p.i_row = i; i += 4 * (rowcap + 1);
+    p.i_misc = i; i += 2 + 8;
+    p.n_ints = i;
+    p.bytes = 8 * p.n_doubles + 4 * p.n_ints;
+  };
+  const int planes = std::max(1, tl->max_planes);
+  layout(std::max(need_rows, 256), std::min(64, planes));
+  if (p.bytes > 48 * 1024) layout(std::max(need_rows, 128), std::min(16, planes));
+  if (p.bytes > 64 * 1024) layout(need_rows, std::min(4, planes));
+  if (p.bytes > 64 * 1024)
+    fatal("task list needs %d bytes of LDS per task (lp=%d, cube half-width %d); "
+          "the limit is 65536", p.bytes, lp, tl->max_cmax);
+  (void)integrate;
+  return p;
+}
+
+int nwaves_for(const TaskListImpl *) {
+  const char *e = getenv("GRID_MI355X_WAVES");
+  const int w = e ? atoi(e) : 2;
+  return (w == 1 || w == 2 || w == 4) ? w : 2;
+}
+
+void set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s) {
+  for (int l = 0; l < tl->nlevels; l++) {
+    Level &L = tl->h_levels[l];
+    L.lay = tl->layouts[l];
+    L.grid = grids[l];
+    L.npts = (long long)L.lay.npts_local[0] * L.lay.npts_local[1] * L.lay.npts_local[2];
+  }
+  HIP_CHECK(hipMemcpyAsync(tl->d_levels.p, tl->h_levels, sizeof(Level) * tl->nlevels,
+                           hipMemcpyHostToDevice, s));
+}
+
+void count_stats(const TaskListImpl *tl, bool collocate, int lpshift) {
+  std::lock_guard<std::mutex> g(S.mu);
+  for (int o = 0; o < 2; o++)
+    for (int l = 0; l < kCounterLp; l++) {
+      if (!tl->hist[o][l]) continue;
+      const int kern = (collocate ? 0 : 1) + (o ? 0 : 2);
+      S.counters[kern][std::min(l + lpshift, kCounterLp - 1)] += tl->hist[o][l];
+    }
+}
+
+int decontract_chunk(const TaskListImpl *tl) {
+  // columns of the Cartesian block per LDS pass: nsb*cach doubles <= 32 KiB
+  const int nsb = std::max(tl->max_nsb, 1);
+  return std::max(1, std::min(std::max(tl->max_ncoa, 1), 4096 / nsb));
+}
+
+void run_decontract(TaskListImpl *tl, const double *pab, hipStream_t s) {
+  const int cach = decontract_chunk(tl);
+  TimedLaunch tm(2, s);
+  HIP_CHECK(launch_decontract((const Group *)tl->d_groups.p, tl->ngroups, pab, 0,
+                              (const double *)tl->d_sphi.p, (double *)tl->d_cart.p,
+                              tl->cart_len, cach, std::max(tl->max_nsb, 1), s));
+}
+
+void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const *grids,
+                   hipStream_t s) {
+  int ld[4];
+  if (!func_ldiffs(func, ld)) fatal("unknown grid_func %d", func);
+  if (precision() != GRID_MI355X_FP64)
+    fatal("FP32-mixed collocation is not available in this build");
+  set_levels(tl, grids, s);
+  {
+    TimedLaunch tm(4, s);
+    for (int l = 0; l < tl->nlevels; l++)
+      HIP_CHECK(launch_fill(grids[l], tl->h_levels[l].npts, 0.0, s));
+  }
+  run_decontract(tl, pab, s);
+  PointParams P{};
+  P.tasks = (const Task *)tl->d_tasks.p;
+  P.ntasks = tl->ntasks;
+  P.groups = (const Group *)tl->d_groups.p;
+  P.levels = (const Level *)tl->d_levels.p;
+  P.cart = (const double *)tl->d_cart.p;
+  P.cart_len = tl->cart_len;
+  P.func = func;
+  for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
+  P.plan = make_plan(tl, ld, false);
+  {
+    TimedLaunch tm(0, s);
+    HIP_CHECK(launch_collocate(P, nwaves_for(tl), s));
+  }
+  count_stats(tl, true, ld[0] + ld[2]);
+}
+
+void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *const *grids,
+                   double *hab, long long hab_len, double *forces, double *virial,
+                   hipStream_t s) {
+  const bool do_f = forces != nullptr, do_v = virial != nullptr;
+  if (do_v && !do_f)
+    fatal("virial requires forces (src/grid/ref/grid_ref_integrate.c:109)");
+  if ((do_f || do_v) && !pab) fatal("forces/virial require pab_blocks");
+  if (precision() != GRID_MI355X_FP64)
+    fatal("FP32-mixed integration is not available in this build");
+  int ld[4];
+  integrate_ldiffs(do_f, do_v, tau, ld);
+  set_levels(tl, (double *const *)grids, s);
+  if (do_f) run_decontract(tl, pab, s);
+  {
+    TimedLaunch tm(4, s);
+    HIP_CHECK(launch_fill((double *)tl->d_hab_cart.p, tl->cart_len, 0.0, s));
+  }
+  PointParams P{};
+  P.tasks = (const Task *)tl->d_tasks.p;
+  P.ntasks = tl->ntasks;
+  P.groups = (const Group *)tl->d_groups.p;
+  P.levels = (const Level *)tl->d_levels.p;
+  P.cart = (const double *)tl->d_cart.p;
+  P.hab_cart = (double *)tl->d_hab_cart.p;
+  P.task_out = (double *)tl->d_task_out.p;
+  P.cart_len = tl->cart_len;
+  for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
+  P.compute_tau = tau;
+  P.do_forces = do_f;
+  P.do_virial = do_v;
+  P.plan = make_plan(tl, ld, true);
+  const int lpmax = P.plan.np - 1;
+  {
+    TimedLaunch tm(1, s);
+    HIP_CHECK(launch_integrate(P, lpmax, nwaves_for(tl), s));
+  }
+  {
+    TimedLaunch tm(4, s);
+    HIP_CHECK(launch_fill(hab, hab_len, 0.0, s));
+  }
+  {
+    TimedLaunch tm(3, s);
+    const int cach = decontract_chunk(tl);
+    HIP_CHECK(launch_contract((const Group *)tl->d_groups.p, tl->ngroups,
+                              (const double *)tl->d_hab_cart.p, (const double *)tl->d_sphi.p,
+                              hab, hab_len, cach, std::max(tl->max_nsb, 1),
+                              tl->max_nsa_nsb, s));
+  }
+  if (do_f || do_v) {
+    TimedLaunch tm(5, s);
+    if (do_f)
+      HIP_CHECK(launch_reduce_forces((const int *)tl->d_fptr.p, (const int *)tl->d_fent.p,
+                                     (const double *)tl->d_task_out.p, forces, tl->natoms, s));
+    if (do_v)
+      HIP_CHECK(launch_reduce_virial((const double *)tl->d_task_out.p, tl->ntasks, virial, s));
+  }
+  count_stats(tl, false, ld[0] + ld[2]);
+}
+
+// Every matrix block addressed by the list must lie inside the buffer.
+void check_buffer(const TaskListImpl *tl, const offload_buffer *b, const char *what) {
+  if (!b) fatal("%s is NULL", what);
+  if ((long long)(b->size / sizeof(double)) < tl->block_end)
+    fatal("%s holds %zu doubles but the task list addresses %lld", what,
+          b->size / sizeof(double), tl->block_end);
+}
+
+double *device_mirror(const offload_buffer *b, DevBuf &scratch) {
+  if (b->device_buffer) return b->device_buffer;
+  return (double *)scratch.get(b->size ? b->size : sizeof(double));
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// ---- offload_buffer, src/offload/offload_buffer.c:52-118 ----
+void offload_create_buffer(const int length, offload_buffer **buffer) {
+  const size_t requested = sizeof(double) * (size_t)(length > 0 ? length : 0);
+  if (*buffer != nullptr) {
+    if (requested == 0) {
+      if ((*buffer)->host_buffer) (void)hipHostFree((*buffer)->host_buffer);
+      if ((*buffer)->device_buffer) (void)hipFree((*buffer)->device_buffer);
+      (*buffer)->size = 0;
+      (*buffer)->host_buffer = nullptr;
+      (*buffer)->device_buffer = nullptr;
+      return;
+    }
+    if (requested <= (*buffer)->size) return;
+    if ((*buffer)->host_buffer) (void)hipHostFree((*buffer)->host_buffer);
+    if ((*buffer)->device_buffer) (void)hipFree((*buffer)->device_buffer);
+  } else {
+    *buffer = (offload_buffer *)malloc(sizeof(offload_buffer));
+    if (!*buffer) fatal("out of host memory");
+  }
+  (*buffer)->size = requested;
+  (*buffer)->host_buffer = nullptr;
+  (*buffer)->device_buffer = nullptr;
+  if (requested == 0) return;
+  ensure_device();
+  HIP_CHECK(hipHostMalloc((void **)&(*buffer)->host_buffer, requested));
+  HIP_CHECK(hipMalloc((void **)&(*buffer)->device_buffer, requested));
+}
+
+void offload_free_buffer(offload_buffer *buffer) {
+  if (!buffer) return;
+  if (buffer->host_buffer) (void)hipHostFree(buffer->host_buffer);
+  if (buffer->device_buffer) (void)hipFree(buffer->device_buffer);
+  free(buffer);
+}
+
+double *offload_get_buffer_host_pointer(offload_buffer *buffer) {
+  return buffer ? buffer->host_buffer : nullptr;
+}
+
+// ---- basis sets, src/grid/common/grid_basis_set.c:19-86 ----
+void grid_create_basis_set(const int nset, const int nsgf, const int maxco, const int maxpgf,
+                           const int *lmin, const int *lmax, const int *npgf,
+                           const int *nsgf_set, const int *first_sgf, const double *sphi,
+                           const double *zet, grid_basis_set **basis_set) {
+  grid_basis_set *b = (grid_basis_set *)malloc(sizeof(grid_basis_set));
+  if (!b) fatal("out of host memory");
+  b->nset = nset;
+  b->nsgf = nsgf;
+  b->maxco = maxco;
+  b->maxpgf = maxpgf;
+  auto icopy = [&](const int *src) {
+    int *d = (int *)malloc(sizeof(int) * (nset > 0 ? nset : 1));
+    if (nset > 0) memcpy(d, src, sizeof(int) * nset);
+    return d;
+  };
+  b->lmin = icopy(lmin);
+  b->lmax = icopy(lmax);
+  b->npgf = icopy(npgf);
+  b->nsgf_set = icopy(nsgf_set);
+  b->first_sgf = icopy(first_sgf);
+  const size_t ns = (size_t)nsgf * maxco, nz = (size_t)nset * maxpgf;
+  b->sphi = (double *)malloc(sizeof(double) * (ns ? ns : 1));
+  if (ns) memcpy(b->sphi, sphi, sizeof(double) * ns);
+  b->zet = (double *)malloc(sizeof(double) * (nz ? nz : 1));
+  if (nz) memcpy(b->zet, zet, sizeof(double) * nz);
+  *basis_set = b;
+}
+
+void grid_free_basis_set(grid_basis_set *b) {
+  if (!b) return;
+  free(b->lmin);
+  free(b->lmax);
+  free(b->npgf);
+  free(b->nsgf_set);
+  free(b->first_sgf);
+  free(b->sphi);
+  free(b->zet);
+  free(b);
+}
+
+// ---- library, src/grid/common/grid_library.c:53-257 ----
+void grid_library_init(void) {
+  std::lock_guard<std::mutex> g(S.mu);
+  if (S.initialized) {
+    printf("Error: Grid library was already initialized.\n");
+    abort();
+  }
+  memset(S.counters, 0, sizeof(S.counters));
+  S.initialized = true;
+}
+
+void grid_library_finalize(void) {
+  std::lock_guard<std::mutex> g(S.mu);
+  if (!S.initialized) {
+    printf("Error: Grid library is not initialized.\n");
+    abort();
+  }
+  S.initialized = false;
+}
+
+void grid_library_set_config(const enum grid_backend backend, const bool validate,
+                             const bool apply_cutoff) {
+  S.config.backend = backend;
+  S.config.validate = validate;
+  S.config.apply_cutoff = apply_cutoff;
+}
+
+grid_library_config grid_library_get_config(void) { return S.config; }
+
+void grid_library_counter_add(const int lp, const enum grid_backend backend,
+                              const enum grid_library_kernel kern, const int increment) {
+  if (lp < 0 || (int)kern < 0 || (int)kern > 3) fatal("bad counter index");
+  (void)backend;  // every counter of this library belongs to the GPU backend
+  std::lock_guard<std::mutex> g(S.mu);
+  S.counters[kern][std::min(lp, kCounterLp - 1)] += increment;
+}
+
+void grid_library_print_stats(const int fortran_comm, void (*print_func)(const char *, int, int),
+                              int output_unit) {
+  (void)fortran_comm;
+  if (!S.initialized) {
+    printf("Error: Grid library is not initialized.\n");
+    abort();
+  }
+  struct Row {
+    long n;
+    int kern, lp;
+  };
+  std::vector<Row> rows;
+  double total = 0.0;
+  {
+    std::lock_guard<std::mutex> g(S.mu);
+    for (int k = 0; k < 4; k++)
+      for (int l = 0; l < kCounterLp; l++)
+        if (S.counters[k][l]) {
+          rows.push_back({S.counters[k][l], k, l});
+          total += S.counters[k][l];
+        }
+  }
+  if (rows.empty()) return;
+  std::stable_sort(rows.begin(), rows.end(), [](const Row &a, const Row &b) { return a.n > b.n; });
+  auto out = [&](const char *m) { print_func(m, (int)strlen(m), output_unit); };
+  const char *rule =
+      " -------------------------------------------------------------------------------\n";
+  out("\n");
+  out(rule);
+  out(" -                                                                             -\n");
+  out(" -                                GRID STATISTICS                              -\n");
+  out(" -                                                                             -\n");
+  out(rule);
+  out(" LP    KERNEL             BACKEND                              COUNT     PERCENT\n");
+  const char *names[] = {"collocate ortho", "integrate ortho", "collocate general",
+                         "integrate general"};
+  for (const Row &r : rows) {
+    char buf[128];
+    snprintf(buf, sizeof(buf), " %-5i %-17s  %-6s  %34li %10.2f%%\n", r.lp, names[r.kern],
+             "GPU", r.n, 100.0 * r.n / total);
+    out(buf);
+  }
+  out(rule);
+}
+
+// ---- task list, src/grid/grid_task_list.c:23-436 ----
+void grid_create_task_list(
+    const bool orthorhombic, const int ntasks, const int nlevels, const int natoms,
+    const int nkinds, const int nblocks, const int *block_offsets,
+    const double (*atom_positions)[3], const int *atom_kinds,
+    const grid_basis_set **basis_sets, const int *level_list, const int *iatom_list,
+    const int *jatom_list, const int *iset_list, const int *jset_list, const int *ipgf_list,
+    const int *jpgf_list, const int *border_mask_list, const int *block_num_list,
+    const double *radius_list, const double (*rab_list)[3], const int (*npts_global)[3],
+    const int (*npts_local)[3], const int (*shift_local)[3], const int (*border_width)[3],
+    const double (*dh)[3][3], const double (*dh_inv)[3][3], grid_task_list **task_list) {
+  grid_gpu_create_task_list(orthorhombic, ntasks, nlevels, natoms, nkinds, nblocks,
+                            block_offsets, (const double *)atom_positions, atom_kinds,
+                            basis_sets, level_list, iatom_list, jatom_list, iset_list,
+                            jset_list, ipgf_list, jpgf_list, border_mask_list, block_num_list,
+                            radius_list, (const double *)rab_list, (const int *)npts_global,
+                            (const int *)npts_local, (const int *)shift_local,
+                            (const int *)border_width, (const double *)dh,
+                            (const double *)dh_inv, (grid_gpu_task_list **)task_list);
+}
+
+void grid_free_task_list(grid_task_list *task_list) { grid_gpu_free_task_list(task_list); }
+
+static void check_npts(const TaskListImpl *tl, int nlevels, const int *npts_local) {
+  if (tl->nlevels != nlevels) fatal("nlevels %d does not match the task list (%d)", nlevels, tl->nlevels);
+  for (int i = 0; i < 3 * nlevels; i++)
+    if (tl->npts_local[i] != npts_local[i])
+      fatal("npts_local of level %d does not match the task list", i / 3 + 1);
+}
+
+void grid_collocate_task_list(const grid_task_list *task_list, const enum grid_func func,
+                              const int nlevels, const int (*npts_local)[3],
+                              const offload_buffer *pab_blocks, offload_buffer **grids) {
+  if (!task_list) return;
+  TaskListImpl *tl = checked(task_list);
+  if (tl->empty) {  // src/grid/grid_task_list.c:185-189
+    for (int l = 0; l < nlevels; l++) memset(grids[l]->host_buffer, 0, grids[l]->size);
+    return;
+  }
+  check_npts(tl, nlevels, (const int *)npts_local);
+  grid_gpu_collocate_task_list(task_list, func, nlevels, pab_blocks, grids);
+}
+
+void grid_integrate_task_list(const grid_task_list *task_list, const bool compute_tau,
+                              const int natoms, const int nlevels, const int (*npts_local)[3],
+                              const offload_buffer *pab_blocks, const offload_buffer **grids,
+                              offload_buffer *hab_blocks, double (*forces)[3],
+                              double (*virial)[3]) {
+  if (!task_list) return;
+  TaskListImpl *tl = checked(task_list);
+  if (tl->empty) {  // src/grid/grid_task_list.c:290-311
+    memset(hab_blocks->host_buffer, 0, hab_blocks->size);
+    if (virial) memset(virial, 0, 9 * sizeof(double));
+    if (forces) memset(forces, 0, 3 * sizeof(double) * (size_t)natoms);
+    return;
+  }
+  check_npts(tl, nlevels, (const int *)npts_local);
+  if (tl->natoms != natoms) fatal("natoms %d does not match the task list (%d)", natoms, tl->natoms);
+  grid_gpu_integrate_task_list(task_list, compute_tau, nlevels, pab_blocks, grids, hab_blocks,
+                               (double *)forces, (double *)virial);
+}
+
+// ---- backend level, src/grid/gpu/grid_gpu_task_list.h:25-59 ----
+void grid_gpu_create_task_list(
+    const bool ortho, const int ntasks, const int nlevels, const int natoms, const int nkinds,
+    const int nblocks, const int *block_offsets, const double *atom_positions,
+    const int *atom_kinds, const grid_basis_set **basis_sets, const int *level_list,
+    const int *iatom_list, const int *jatom_list, const int *iset_list, const int *jset_list,
+    const int *ipgf_list, const int *jpgf_list, const int *border_mask_list,
+    const int *block_num_list, const double *radius_list, const double *rab_list,
+    const int *npts_global, const int *npts_local, const int *shift_local,
+    const int *border_width, const double *dh, const double *dh_inv,
+    grid_gpu_task_list **ptr) {
+  TaskListImpl *tl;
+  if (*ptr) {
+    // reuse the handle (src/grid/grid_task_list.c:42-63): rebuild in place
+    TaskListImpl *old = checked(*ptr);
+    old->~TaskListImpl();
+    tl = new (old) TaskListImpl();
+  } else {
+    tl = new TaskListImpl();
+  }
+  build_task_list(tl, ortho, ntasks, nlevels, natoms, nkinds, nblocks, block_offsets,
+                  atom_positions, atom_kinds, basis_sets, level_list, iatom_list, jatom_list,
+                  iset_list, jset_list, ipgf_list, jpgf_list, border_mask_list, block_num_list,
+                  radius_list, rab_list, npts_global, npts_local, shift_local, border_width, dh,
+                  dh_inv);
+  *ptr = tl;
+}
+
+void grid_gpu_free_task_list(grid_gpu_task_list *ptr) {
+  if (!ptr) return;
+  TaskListImpl *tl = checked(ptr);
+  delete tl;
+}
+
+void grid_gpu_collocate_task_list(const grid_gpu_task_list *ptr, const enum grid_func func,
+                                  const int nlevels, const offload_buffer *pab_blocks,
+                                  offload_buffer **grids) {
+  TaskListImpl *tl = checked(ptr);
+  if (tl->empty) {
+    for (int l = 0; l < nlevels; l++) memset(grids[l]->host_buffer, 0, grids[l]->size);
+    return;
+  }
+  if (nlevels != tl->nlevels) fatal("nlevels does not match the task list");
+  const hipStream_t s = stream();
+  check_buffer(tl, pab_blocks, "pab_blocks");
+  double *pab = device_mirror(pab_blocks, tl->pab_dev);
+  HIP_CHECK(hipMemcpyAsync(pab, pab_blocks->host_buffer, pab_blocks->size,
+                           hipMemcpyHostToDevice, s));
+  std::vector<double *> g(nlevels);
+  for (int l = 0; l < nlevels; l++) {
+    const size_t need = sizeof(double) * (size_t)tl->npts_local[3 * l] *
+                        tl->npts_local[3 * l + 1] * tl->npts_local[3 * l + 2];
+    if (grids[l]->size < need) fatal("grid buffer of level %d is too small", l + 1);
+    g[l] = device_mirror(grids[l], tl->grid_dev[l]);
+  }
+  run_collocate(tl, func, pab, g.data(), s);
+  for (int l = 0; l < nlevels; l++)
+    HIP_CHECK(hipMemcpyAsync(grids[l]->host_buffer, g[l], grids[l]->size,
+                             hipMemcpyDeviceToHost, s));
+  sync_stream(s);
+}
+
+void grid_gpu_integrate_task_list(const grid_gpu_task_list *ptr, const bool compute_tau,
+                                  const int nlevels, const offload_buffer *pab_blocks,
+                                  const offload_buffer **grids, offload_buffer *hab_blocks,
+                                  double *forces, double *virial) {
+  TaskListImpl *tl = checked(ptr);
+  if (tl->empty) {
+    memset(hab_blocks->host_buffer, 0, hab_blocks->size);
+    if (virial) memset(virial, 0, 9 * sizeof(double));
+    if (forces) memset(forces, 0, 3 * sizeof(double) * (size_t)tl->natoms);
+    return;
+  }
+  if (nlevels != tl->nlevels) fatal("nlevels does not match the task list");
+  const hipStream_t s = stream();
+  std::vector<const double *> g(nlevels);
+  for (int l = 0; l < nlevels; l++) {
+    const size_t need = sizeof(double) * (size_t)tl->npts_local[3 * l] *
+                        tl->npts_local[3 * l + 1] * tl->npts_local[3 * l + 2];
+    if (grids[l]->size < need) fatal("grid buffer of level %d is too small", l + 1);
+    double *d = device_mirror(grids[l], tl->grid_dev[l]);
+    HIP_CHECK(hipMemcpyAsync(d, grids[l]->host_buffer, need, hipMemcpyHostToDevice, s));
+    g[l] = d;
+  }
+  const double *pab = nullptr;
+  check_buffer(tl, hab_blocks, "hab_blocks");
+  if (pab_blocks && (forces || virial)) {
+    check_buffer(tl, pab_blocks, "pab_blocks");
+    double *d = device_mirror(pab_blocks, tl->pab_dev);
+    HIP_CHECK(hipMemcpyAsync(d, pab_blocks->host_buffer, pab_blocks->size,
+                             hipMemcpyHostToDevice, s));
+    pab = d;
+  }
+  double *hab = device_mirror(hab_blocks, tl->hab_dev);
+  const long long hab_len = (long long)(hab_blocks->size / sizeof(double));
+  double *fv = (double *)tl->forces_dev.get(sizeof(double) * (3 * (size_t)tl->natoms + 9));
+  run_integrate(tl, compute_tau, pab, g.data(), hab, hab_len, forces ? fv : nullptr,
+                virial ? fv + 3 * tl->natoms : nullptr, s);
+  HIP_CHECK(hipMemcpyAsync(hab_blocks->host_buffer, hab, hab_blocks->size,
+                           hipMemcpyDeviceToHost, s));
+  if (forces)
+    HIP_CHECK(hipMemcpyAsync(forces, fv, sizeof(double) * 3 * tl->natoms,
+                             hipMemcpyDeviceToHost, s));
+  if (virial)
+    HIP_CHECK(hipMemcpyAsync(virial, fv + 3 * tl->natoms, sizeof(double) * 9,
+                             hipMemcpyDeviceToHost, s));
+  sync_stream(s);
+}
+
+// ---- extensions, include/grid_mi355x.h ----
+void grid_mi355x_set_precision(int p) {
+  if (p != GRID_MI355X_FP64 && p != GRID_MI355X_FP32_MIXED) fatal("unknown precision %d", p);
+  S.precision = p;
+}
+
+int grid_mi355x_get_precision(void) { return precision(); }
+
+void grid_mi355x_collocate_task_list_dev(const grid_task_list *task_list,
+                                         const enum grid_func func, const int nlevels,
+                                         const double *pab_blocks, double *const *grids) {
+  TaskListImpl *tl = checked(task_list);
+  const hipStream_t s = stream();
+  if (tl->empty) {
+    for (int l = 0; l < nlevels && l < (int)tl->npts_local.size() / 3; l++)
+      HIP_CHECK(launch_fill(grids[l], (long long)tl->npts_local[3 * l] * tl->npts_local[3 * l + 1] *
+                                          tl->npts_local[3 * l + 2], 0.0, s));
+    return;
+  }
+  if (nlevels != tl->nlevels) fatal("nlevels does not match the task list");
+  run_collocate(tl, func, pab_blocks, grids, s);
+}
+
+void grid_mi355x_integrate_task_list_dev(const grid_task_list *task_list, const bool compute_tau,
+                                         const int natoms, const int nlevels,
+                                         const double *pab_blocks, const double *const *grids,
+                                         double *hab_blocks, const long hab_length,
+                                         double *forces, double *virial) {
+  TaskListImpl *tl = checked(task_list);
+  const hipStream_t s = stream();
+  if (tl->empty) {
+    HIP_CHECK(launch_fill(hab_blocks, hab_length, 0.0, s));
+    if (forces) HIP_CHECK(launch_fill(forces, 3LL * natoms, 0.0, s));
+    if (virial) HIP_CHECK(launch_fill(virial, 9, 0.0, s));
+    return;
+  }
+  if (nlevels != tl->nlevels) fatal("nlevels does not match the task list");
+  if (natoms != tl->natoms) fatal("natoms does not match the task list");
+  if (hab_length < tl->block_end)
+    fatal("hab_blocks holds %ld doubles but the task list addresses %lld", hab_length,
+          tl->block_end);
+  run_integrate(tl, compute_tau, pab_blocks, grids, hab_blocks, hab_length, forces, virial, s);
+}
+
+void grid_mi355x_set_stream(void *hip_stream) {
+  S.user_stream = (hipStream_t)hip_stream;
+  S.stream_set = hip_stream != nullptr;
+}
+
+void *grid_mi355x_get_stream(void) { return (void *)stream(); }
+
+void grid_mi355x_synchronize(void) { sync_stream(stream()); }
+
+void grid_mi355x_set_timing(const bool enable) {
+  if (!enable) resolve_timings();
+  S.timing = enable;
+}
+
+void grid_mi355x_reset_timing(void) {
+  resolve_timings();
+  for (int i = 0; i < kNumKernelIds; i++) {
+    S.kernel_ms[i] = 0.0;
+    S.kernel_n[i] = 0;
+  }
+}
+
+double grid_mi355x_kernel_time_ms(const int kernel_id, long *launches) {
+  if (kernel_id < 0 || kernel_id >= kNumKernelIds) fatal("unknown kernel id %d", kernel_id);
+  resolve_timings();
+  if (launches) *launches = S.kernel_n[kernel_id];
+  return S.kernel_ms[kernel_id];
+}
+
+long long grid_mi355x_task_points(const grid_task_list *task_list) {
+  TaskListImpl *tl = checked(task_list);
+  if (tl->empty) return 0;
+  if (tl->points >= 0) return tl->points;
+  const hipStream_t s = stream();
+  std::vector<double *> g(tl->nlevels, nullptr);
+  set_levels(tl, g.data(), s);
+  PointParams P{};
+  P.tasks = (const Task *)tl->d_tasks.p;
+  P.ntasks = tl->ntasks;
+  P.groups = (const Group *)tl->d_groups.p;
+  P.levels = (const Level *)tl->d_levels.p;
+  P.task_out = (double *)tl->d_task_out.p;
+  P.count_only = 1;
+  const int ld[4] = {0, 0, 0, 0};
+  P.plan = make_plan(tl, ld, false);
+  HIP_CHECK(launch_collocate(P, nwaves_for(tl), s));
+  std::vector<double> out((size_t)tl->ntasks * kTaskOutStride);
+  HIP_CHECK(hipMemcpyAsync(out.data(), tl->d_task_out.p, sizeof(double) * out.size(),
+                           hipMemcpyDeviceToHost, s));
+  sync_stream(s);
+  long long total = 0;
+  for (int t = 0; t < tl->ntasks; t++) total += (long long)out[(size_t)t * kTaskOutStride];
+  tl->points = total;
+  return total;
+}
+
+void grid_mi355x_task_counts(const grid_task_list *task_list, long *ortho, long *general,
+                             long *skipped) {
+  const TaskListImpl *tl = checked(task_list);
+  if (ortho) *ortho = tl->n_ortho;
+  if (general) *general = tl->n_general;
+  if (skipped) *skipped = tl->n_skip;
+}
+
+}  // extern "C"

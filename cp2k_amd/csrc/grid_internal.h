@@ -1,0 +1,100 @@
+// Internal data layout shared by the host side (grid_api.cpp) and the HIP
+// kernels (grid_kernels.hip).  Everything geometric is decided once, in FP64,
+// when the task list is created; a collocate/integrate call only moves the
+// density/potential data and launches kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace gm {
+
+constexpr int kMaxL = 16;            // orbital table covers l <= kMaxL
+constexpr int kNumOrb = (kMaxL + 1) * (kMaxL + 2) * (kMaxL + 3) / 6;
+constexpr int kTaskOutStride = 15;   // fa[3], fb[3], virial[9] per task
+
+// Per-level grid layout, src/grid/grid_task_list.h:46-53.
+struct Layout {
+  int npts_global[3];
+  int npts_local[3];
+  int shift_local[3];
+  int border_width[3];
+  double dh[3][3];
+  double dh_inv[3][3];
+};
+
+struct Level {
+  Layout lay;
+  double *grid;         // device pointer of this call's level grid
+  long long npts;       // nx*ny*nz of the local grid (bounds guard)
+};
+
+enum : int { kTaskOrtho = 1, kTaskSkip = 2 };
+
+// One task = one product of primitive Gaussians on one grid level.
+// Orthorhombic tasks (orthorhombic cell and border_mask == 0) use center,
+// roff, disr, lo (= lb_cube, ub = 1 - lo) and cmax; general tasks use gp,
+// lo/hi (= index box) and blo/bhi (border-masked bounds of the local grid).
+struct Task {
+  double ra[3], rab[3], rp[3], rb[3];
+  double zeta, zetb, zetp, pexp;  // pexp = exp(-zeta*zetb/zetp*|rab|^2)
+  double rscale;                  // 1 on-site, 2 off-site (collocate only)
+  double radius, disr;
+  double roff[3];
+  double gp[3];
+  int center[3];
+  int lo[3], hi[3];
+  int blo[3], bhi[3];
+  int cmax;
+  int la_max, la_min, lb_max, lb_min;
+  int level, flags, group;
+  int o1, o2;          // Cartesian offsets of this pgf pair in the group block
+  int iatom, jatom;    // zero based
+  int pad_;
+};
+
+// One (matrix block, iset, jset) sub-block: the unit of (de)contraction.
+struct Group {
+  long long cart_off;  // [ncob][ncoa] Cartesian block in scratch
+  long long sphi_a;    // offset of sphi row sa0 of the i-kind (row stride maxcoa)
+  long long sphi_b;
+  int block_off;       // offset of the matrix block in the pab/hab buffer
+  int transpose;       // iatom <= jatom: block stored [sgf_b][sgf_a]
+  int ncoa, ncob;
+  int nsa, nsb;        // spherical functions of the two sets
+  int sa0, sb0;        // first spherical function of the sets (zero based)
+  int nsgfa, nsgfb;    // block dimensions
+  int maxcoa, maxcob;
+  int iatom, jatom;
+};
+
+// LDS carve-up of the per-task kernels, decided on the host from the
+// maxima over the launched tasks (offsets in doubles / ints).
+struct LdsPlan {
+  int np;        // stride for polynomial index: max lp + 1
+  int ext;       // stride of the per-axis tables: 2*max cmax + 2
+  int mapn;      // ints per map array
+  int rowcap, plcap;
+  int o_cab, o_alpha, o_cxyz, o_cijk, o_tab, o_cxy, o_cxrow, o_red;  // doubles
+  int n_doubles;
+  int i_map, i_row, i_plane, i_misc;  // ints, after the doubles
+  int n_ints;
+  int bytes;
+};
+
+struct PointParams {
+  const Task *tasks;
+  int ntasks;
+  const Group *groups;
+  const Level *levels;
+  const double *cart;     // decontracted pab (collocate; integrate with forces)
+  double *hab_cart;       // integrate: Cartesian hab accumulation
+  double *task_out;       // integrate: per-task forces/virial
+  long long cart_len;
+  int func;
+  int ld[4];              // l-range shifts: la_max, la_min, lb_max, lb_min
+  int compute_tau, do_forces, do_virial;
+  int count_only;         // count task points into task_out[t*15]
+  LdsPlan plan;
+};
+
+}  // namespace gm

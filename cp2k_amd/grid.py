@@ -1,0 +1,268 @@
+"""Python binding of libcp2k_grid_mi355x.so, shaped like CP2K's grid_api.F.
+
+`GridTaskList` mirrors the Fortran module's task-list calls
+(src/grid/grid_api.F:721-1129): create from the per-task lists, collocate a
+density (per grid level), integrate a potential into Hamiltonian blocks,
+forces and virial.  Calls go through the library's C-ABI exactly as CP2K's
+ISO_C_BINDING interfaces do.  There is no fallback: if the HIP library is not
+built, or no GPU is visible, the calls fail.
+"""
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from .build import LIB as LIB_PATH
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+# exported by the library; checked by tests/test_abi.py against include/*.h
+EXPORTS = [
+    "grid_create_task_list", "grid_free_task_list", "grid_collocate_task_list",
+    "grid_integrate_task_list", "grid_create_basis_set", "grid_free_basis_set",
+    "grid_library_init", "grid_library_finalize", "grid_library_set_config",
+    "grid_library_get_config", "grid_library_print_stats",
+    "grid_library_counter_add", "offload_create_buffer", "offload_free_buffer",
+    "offload_get_buffer_host_pointer", "grid_gpu_create_task_list",
+    "grid_gpu_free_task_list", "grid_gpu_collocate_task_list",
+    "grid_gpu_integrate_task_list", "grid_mi355x_set_precision",
+    "grid_mi355x_get_precision", "grid_mi355x_collocate_task_list_dev",
+    "grid_mi355x_integrate_task_list_dev", "grid_mi355x_set_stream",
+    "grid_mi355x_get_stream", "grid_mi355x_synchronize",
+    "grid_mi355x_set_timing", "grid_mi355x_reset_timing",
+    "grid_mi355x_kernel_time_ms", "grid_mi355x_task_points",
+    "grid_mi355x_task_counts",
+]
+
+GRID_FUNC_AB = 100
+GRID_FUNC_DADB = 200
+FP64 = 0
+FP32_MIXED = 1
+
+KERNEL_COLLOCATE, KERNEL_INTEGRATE, KERNEL_DECONTRACT, KERNEL_CONTRACT, \
+    KERNEL_FILL, KERNEL_REDUCE = range(6)
+
+
+class GridBasisSet(C.Structure):
+    """grid_basis_set, src/grid/common/grid_basis_set.h:14-26."""
+    _fields_ = [("nset", C.c_int), ("nsgf", C.c_int), ("maxco", C.c_int),
+                ("maxpgf", C.c_int), ("lmin", _ip), ("lmax", _ip),
+                ("npgf", _ip), ("nsgf_set", _ip), ("first_sgf", _ip),
+                ("sphi", _dp), ("zet", _dp)]
+
+
+class OffloadBuffer(C.Structure):
+    """offload_buffer, src/offload/offload_buffer.h:16-20."""
+    _fields_ = [("size", C.c_size_t), ("host_buffer", _dp),
+                ("device_buffer", _dp)]
+
+
+class LibraryConfig(C.Structure):
+    _fields_ = [("backend", C.c_int), ("validate", C.c_bool),
+                ("apply_cutoff", C.c_bool)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not pathlib.Path(LIB_PATH).exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(str(LIB_PATH))
+        L.grid_create_basis_set.restype = None
+        L.grid_free_basis_set.argtypes = [C.c_void_p]
+        L.grid_create_task_list.restype = None
+        L.grid_free_task_list.argtypes = [C.c_void_p]
+        L.grid_collocate_task_list.restype = None
+        L.grid_integrate_task_list.restype = None
+        L.grid_library_get_config.restype = LibraryConfig
+        L.offload_create_buffer.argtypes = [C.c_int, C.POINTER(C.POINTER(OffloadBuffer))]
+        L.offload_free_buffer.argtypes = [C.POINTER(OffloadBuffer)]
+        L.grid_mi355x_kernel_time_ms.restype = C.c_double
+        L.grid_mi355x_kernel_time_ms.argtypes = [C.c_int, C.POINTER(C.c_long)]
+        L.grid_mi355x_task_points.restype = C.c_longlong
+        L.grid_mi355x_task_points.argtypes = [C.c_void_p]
+        L.grid_mi355x_get_stream.restype = C.c_void_p
+        L.grid_mi355x_set_stream.argtypes = [C.c_void_p]
+        L.grid_mi355x_set_timing.argtypes = [C.c_bool]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Basis:
+    """A grid_basis_set created through the library."""
+
+    def __init__(self, spec):
+        L = lib()
+        self.spec = spec
+        self._arrays = [np.ascontiguousarray(x, dtype=np.int32) for x in
+                        (spec.lmin, spec.lmax, spec.npgf, spec.nsgf_set, spec.first_sgf)]
+        self._sphi = np.ascontiguousarray(spec.sphi, dtype=np.float64)
+        self._zet = np.ascontiguousarray(spec.zet, dtype=np.float64)
+        self.h = C.POINTER(GridBasisSet)()
+        L.grid_create_basis_set(
+            C.c_int(spec.nset), C.c_int(spec.nsgf), C.c_int(spec.maxco),
+            C.c_int(spec.maxpgf), *[_ptr(a, C.c_int) for a in self._arrays],
+            _ptr(self._sphi), _ptr(self._zet), C.byref(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().grid_free_basis_set(self.h)
+            self.h = None
+
+
+class OffloadBuf:
+    """An offload_buffer (pinned host + HBM mirror) created by the library."""
+
+    def __init__(self, length):
+        self.h = C.POINTER(OffloadBuffer)()
+        lib().offload_create_buffer(C.c_int(int(length)), C.byref(self.h))
+        self.length = int(length)
+
+    def host(self):
+        if self.length == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(self.h.contents.host_buffer, shape=(self.length,))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().offload_free_buffer(self.h)
+            self.h = None
+
+
+def set_precision(p):
+    lib().grid_mi355x_set_precision(C.c_int(int(p)))
+
+
+def get_precision():
+    return int(lib().grid_mi355x_get_precision())
+
+
+class GridTaskList:
+    """A task list on the GPU, created from a TaskListArgs."""
+
+    def __init__(self, args):
+        L = lib()
+        self.args = a = args
+        self._bases = [Basis(b) for b in a.basis_sets]
+        self._basis_arr = (C.POINTER(GridBasisSet) * len(self._bases))(
+            *[b.h for b in self._bases])
+        self.h = C.c_void_p()
+        L.grid_create_task_list(
+            C.c_bool(a.orthorhombic), C.c_int(a.ntasks), C.c_int(a.nlevels),
+            C.c_int(a.natoms), C.c_int(a.nkinds), C.c_int(a.nblocks),
+            _ptr(a.block_offsets, C.c_int), _ptr(a.atom_positions),
+            _ptr(a.atom_kinds, C.c_int), self._basis_arr,
+            _ptr(a.level_list, C.c_int), _ptr(a.iatom_list, C.c_int),
+            _ptr(a.jatom_list, C.c_int), _ptr(a.iset_list, C.c_int),
+            _ptr(a.jset_list, C.c_int), _ptr(a.ipgf_list, C.c_int),
+            _ptr(a.jpgf_list, C.c_int), _ptr(a.border_mask_list, C.c_int),
+            _ptr(a.block_num_list, C.c_int), _ptr(a.radius_list),
+            _ptr(a.rab_list), _ptr(a.npts_global, C.c_int),
+            _ptr(a.npts_local, C.c_int), _ptr(a.shift_local, C.c_int),
+            _ptr(a.border_width, C.c_int), _ptr(a.dh), _ptr(a.dh_inv),
+            C.byref(self.h))
+        self._npts = np.ascontiguousarray(a.npts_local, dtype=np.int32)
+
+    def free(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().grid_free_task_list(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.free()
+
+    # ---- host-buffer API (the CP2K ABI) ----
+    def collocate(self, func, pab_blocks):
+        """grid_collocate_task_list: returns one [nz, ny, nx] array per level."""
+        a = self.args
+        pab = OffloadBuf(a.buffer_size)
+        pab.host()[:] = pab_blocks
+        grids = [OffloadBuf(int(np.prod(s))) for s in a.grid_shapes()]
+        arr = (C.POINTER(OffloadBuffer) * a.nlevels)(*[g.h for g in grids])
+        lib().grid_collocate_task_list(self.h, C.c_int(func), C.c_int(a.nlevels),
+                                       _ptr(self._npts, C.c_int), pab.h, arr)
+        return [g.host().reshape(s).copy() for g, s in zip(grids, a.grid_shapes())]
+
+    def integrate(self, grids, pab_blocks=None, compute_tau=False, forces=False,
+                  virial=False):
+        """grid_integrate_task_list: returns (hab_blocks, forces, virial)."""
+        a = self.args
+        gbufs = []
+        for g, s in zip(grids, a.grid_shapes()):
+            b = OffloadBuf(int(np.prod(s)))
+            b.host()[:] = np.ascontiguousarray(g, dtype=np.float64).ravel()
+            gbufs.append(b)
+        arr = (C.POINTER(OffloadBuffer) * a.nlevels)(*[g.h for g in gbufs])
+        pab = None
+        if pab_blocks is not None:
+            pab = OffloadBuf(a.buffer_size)
+            pab.host()[:] = pab_blocks
+        hab = OffloadBuf(a.buffer_size)
+        f = np.zeros((a.natoms, 3)) if forces else None
+        v = np.zeros((3, 3)) if virial else None
+        lib().grid_integrate_task_list(
+            self.h, C.c_bool(compute_tau), C.c_int(a.natoms), C.c_int(a.nlevels),
+            _ptr(self._npts, C.c_int), pab.h if pab is not None else None, arr,
+            hab.h, _ptr(f), _ptr(v))
+        return hab.host().copy(), f, v
+
+    # ---- HBM-resident API (extension) on torch tensors ----
+    def collocate_dev(self, func, pab, grids):
+        """pab: float64 device tensor; grids: list of float64 device tensors."""
+        a = self.args
+        gp = (C.c_void_p * a.nlevels)(*[g.data_ptr() for g in grids])
+        lib().grid_mi355x_collocate_task_list_dev(
+            self.h, C.c_int(func), C.c_int(a.nlevels), C.c_void_p(pab.data_ptr()), gp)
+
+    def integrate_dev(self, grids, hab, pab=None, forces=None, virial=None,
+                      compute_tau=False):
+        a = self.args
+        gp = (C.c_void_p * a.nlevels)(*[g.data_ptr() for g in grids])
+        lib().grid_mi355x_integrate_task_list_dev(
+            self.h, C.c_bool(compute_tau), C.c_int(a.natoms), C.c_int(a.nlevels),
+            C.c_void_p(pab.data_ptr() if pab is not None else 0), gp,
+            C.c_void_p(hab.data_ptr()), C.c_long(hab.numel()),
+            C.c_void_p(forces.data_ptr() if forces is not None else 0),
+            C.c_void_p(virial.data_ptr() if virial is not None else 0))
+
+    def task_points(self):
+        return int(lib().grid_mi355x_task_points(self.h))
+
+    def task_counts(self):
+        o, g, s = C.c_long(), C.c_long(), C.c_long()
+        lib().grid_mi355x_task_counts(self.h, C.byref(o), C.byref(g), C.byref(s))
+        return o.value, g.value, s.value
+
+
+def set_stream(stream_handle):
+    lib().grid_mi355x_set_stream(C.c_void_p(stream_handle or 0))
+
+
+def synchronize():
+    lib().grid_mi355x_synchronize()
+
+
+def set_timing(enable):
+    lib().grid_mi355x_set_timing(C.c_bool(enable))
+
+
+def reset_timing():
+    lib().grid_mi355x_reset_timing()
+
+
+def kernel_time_ms(kernel_id):
+    n = C.c_long()
+    ms = lib().grid_mi355x_kernel_time_ms(C.c_int(kernel_id), C.byref(n))
+    return ms, n.value

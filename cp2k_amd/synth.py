@@ -1,0 +1,516 @@
+"""Seeded synthetic task lists shaped like the ones CP2K builds.
+
+Task generation restates, in vectorised numpy:
+  - pgf / set / kind radii and pair screening of task_list_inner_loop
+    (src/task_list_methods.F:652-719): a pgf pair is kept when
+    rpgf_a + rpgf_b >= |rab|;
+  - the product radius exp_radius_very_extended + exp_radius
+    (src/aobasis/ao_util.F:95-290), with single-precision exponentials as there;
+  - the grid level rule gaussian_gridlevel (src/pw_env/gaussian_gridlevels.F
+    :146-166): the coarsest level whose cutoff + 1e-6 >= zetp * rel_cutoff.
+Radius and level are inputs of grid_create_task_list, so these restatements
+shape the workload but do not enter the collocate/integrate parity checks.
+
+Configurations (each seeded, printed with `python -m cp2k_amd.synth NAME`):
+  ref32     32 atoms, cubic 10 A, one 64^3 level, primitive s/p/d, 0.1-20
+  h2o64     64 rigid H2O in a 12.4 A cube, 4 levels 128/72/42/24, DZVP-like
+  tri1024   1024 atoms in a triclinic cell (80-100 deg), 4 levels from 200^3,
+            TZV2P-like (l <= 3), exponents 0.03-100
+  hl2048    2048 atoms, orthorhombic, 5 levels from 256^3, l <= 4, 0.02-200
+  h2o<N>, tri<N>, rand...: reduced versions for parity tests
+"""
+import math
+import sys
+
+import numpy as np
+
+from .tasklist import BasisSpec, TaskListArgs, ncoset
+
+ANGSTROM = 1.0 / 0.52917721067  # bohr per angstrom
+
+
+# ---------------------------------------------------------------------------
+# radii (src/aobasis/ao_util.F:95-290)
+# ---------------------------------------------------------------------------
+def _exp32(x):
+    return np.exp(np.asarray(x, dtype=np.float32)).astype(np.float64)
+
+
+def exp_radius(l, alpha, threshold, prefactor, epsabs=1.0e-12, rlow=None):
+    """Vectorised exp_radius: smallest r (to epsabs, upper end of the final
+    bracket) with prefactor * r**l * exp(-alpha r**2) < threshold."""
+    alpha = np.abs(np.broadcast_to(np.asarray(alpha, dtype=np.float64), np.shape(prefactor)))
+    d = np.abs(np.asarray(prefactor, dtype=np.float64))
+    n = d.shape[0] if d.ndim else 1
+    alpha = np.broadcast_to(alpha, d.shape).astype(np.float64).copy()
+    d = d.reshape(-1)
+    alpha = alpha.reshape(-1)
+    base = np.zeros(n) if rlow is None else np.asarray(rlow, dtype=np.float64).reshape(-1).copy()
+    out = base.copy()
+    t = abs(threshold)
+
+    def g(r):
+        v = d[idx] * _exp32(-alpha[idx] * r * r)
+        return v * r ** l if l else v
+
+    idx = np.nonzero(d != 0.0)[0]
+    r = np.maximum(np.sqrt(0.5 * l / alpha[idx]), base[idx])
+    g0 = g(r) if l else d[idx].copy()
+    keep = g0 >= t
+    idx, r = idx[keep], r[keep]
+    radius = r * 2.0 + 1.0
+    for _ in range(5000):
+        if idx.size == 0:
+            break
+        gv = g(radius)
+        grow = gv >= t
+        if not grow.any():
+            break
+        r = np.where(grow, radius, r)
+        radius = np.where(grow, radius * 2.0 + 1.0, radius)
+    dr = np.zeros_like(r)
+    act = np.ones(idx.size, dtype=bool)
+    for _ in range(5000):
+        rd = radius - r
+        act &= ~((rd < epsabs) | (rd == dr))
+        if not act.any():
+            break
+        s1 = r + rd * 0.38
+        s2 = r + rd * 0.62
+        g1, g2 = g(s1), g(s2)
+        c1 = act & (g1 < t)
+        c2 = act & ~c1 & (g2 < t)
+        c3 = act & ~c1 & ~c2
+        radius = np.where(c1, s1, np.where(c2, s2, radius))
+        r = np.where(c2, s1, np.where(c3, s2, r))
+        dr = np.where(act, rd, dr)
+    out[idx] = radius
+    return out
+
+
+def product_radius(la_max, lb_max, zeta, zetb, dab, eps):
+    """exp_radius_very_extended without pab (cutoff = 1): radius of the
+    product Gaussian of two pgfs at distance dab."""
+    zeta = np.broadcast_to(np.asarray(zeta, dtype=np.float64), np.shape(dab))
+    zetb = np.broadcast_to(np.asarray(zetb, dtype=np.float64), np.shape(dab))
+    zetp = zeta + zetb
+    f = zetb / zetp
+    pref = np.exp(-zeta * f * dab * dab)
+    rad_a = f * dab               # |ra - rp|
+    rad_b = (zeta / zetp) * dab   # |rb - rp|
+    lp = la_max + lb_max
+    poly = np.zeros((lp + 1,) + np.shape(dab))
+    for lxa in range(la_max + 1):
+        for lxb in range(lb_max + 1):
+            coef = np.zeros_like(poly)
+            for i in range(lxa + 1):
+                for j in range(lxb + 1):
+                    coef[lxa + lxb - i - j] += (math.comb(lxa, i) * math.comb(lxb, j)
+                                                * rad_a ** i * rad_b ** j)
+            poly = np.maximum(poly, coef)
+    poly *= pref
+    radius = np.zeros(np.shape(dab))
+    for i in range(lp + 1):
+        radius = np.maximum(radius, exp_radius(i, zetp, eps, poly[i], 1.0e-2, rlow=radius))
+    return radius
+
+
+# ---------------------------------------------------------------------------
+# basis sets
+# ---------------------------------------------------------------------------
+def _cart_of_shell(l):
+    """coset indices of shell l (within ncoset(l))."""
+    return list(range(ncoset(l - 1), ncoset(l)))
+
+
+def make_basis(rng, sets, exp_lo, exp_hi, primitive=False):
+    """sets: list of (lmin, lmax, npgf, ncontr). Exponents log-uniform in
+    [exp_lo, exp_hi] (descending within a set).  primitive=True gives one
+    uncontracted Cartesian function per Cartesian orbital (identity sphi)."""
+    nset = len(sets)
+    lmin = np.array([s[0] for s in sets], dtype=np.int32)
+    lmax = np.array([s[1] for s in sets], dtype=np.int32)
+    npgf = np.array([s[2] for s in sets], dtype=np.int32)
+    maxpgf = int(npgf.max())
+    maxco = max(int(npgf[i]) * ncoset(int(lmax[i])) for i in range(nset))
+    zet = np.zeros((nset, maxpgf))
+    rows = []
+    nsgf_set = []
+    for i, (l0, l1, np_, nc) in enumerate(sets):
+        z = np.sort(np.exp(rng.uniform(math.log(exp_lo), math.log(exp_hi), np_)))[::-1]
+        zet[i, :np_] = z
+        nsl = ncoset(l1)
+        set_rows = []
+        for l in range(l0, l1 + 1):
+            cart = _cart_of_shell(l)
+            if primitive:
+                for p in range(np_):
+                    for c in cart:
+                        row = np.zeros(maxco)
+                        row[p * nsl + c] = 1.0
+                        set_rows.append(row)
+                continue
+            nsph = 2 * l + 1
+            for _ in range(nc):
+                coeff = rng.uniform(0.2, 1.0, np_) * rng.choice([-1.0, 1.0], np_)
+                trans = rng.normal(size=(nsph, len(cart))) / math.sqrt(len(cart))
+                for m in range(nsph):
+                    row = np.zeros(maxco)
+                    for p in range(np_):
+                        row[p * nsl + np.array(cart)] = coeff[p] * trans[m]
+                    set_rows.append(row)
+        nsgf_set.append(len(set_rows))
+        rows.extend(set_rows)
+    sphi = np.array(rows).reshape(len(rows), maxco)
+    first = np.cumsum([0] + nsgf_set[:-1]).astype(np.int32) + 1
+    return BasisSpec(lmin=lmin, lmax=lmax, npgf=npgf,
+                     nsgf_set=np.array(nsgf_set, dtype=np.int32), first_sgf=first,
+                     sphi=sphi, zet=zet)
+
+
+def pgf_radii(basis, eps_pgf=1.0e-5):
+    """rpgf(ipgf, iset) = exp_radius(lmax, zet, eps_pgf_orb, 1)."""
+    r = np.zeros(basis.zet.shape)
+    for s in range(basis.nset):
+        n = int(basis.npgf[s])
+        r[s, :n] = exp_radius(int(basis.lmax[s]), basis.zet[s, :n], eps_pgf,
+                              np.ones(n))
+    return r
+
+
+# ---------------------------------------------------------------------------
+# cells and grids
+# ---------------------------------------------------------------------------
+def cubic_cell(L):
+    return np.diag([L, L, L]).astype(np.float64)
+
+
+def triclinic_cell(rng, volume, lo=80.0, hi=100.0):
+    al, be, ga = np.radians(rng.uniform(lo, hi, 3))
+    a = np.array([1.0, 0.0, 0.0])
+    b = np.array([math.cos(ga), math.sin(ga), 0.0])
+    cx = math.cos(be)
+    cy = (math.cos(al) - math.cos(be) * math.cos(ga)) / math.sin(ga)
+    c = np.array([cx, cy, math.sqrt(max(1.0 - cx * cx - cy * cy, 1e-6))])
+    h = np.array([a, b, c])
+    s = (volume / abs(np.linalg.det(h))) ** (1.0 / 3.0)
+    return h * s
+
+
+def grid_levels(cell, finest, nlevels, progression=3.0):
+    """Grid sizes n_i ~ finest / sqrt(progression)^(i-1), rounded to even
+    2,3,5,7-smooth numbers; cutoffs in Ry = (pi/dh)^2 of each level."""
+    def smooth(n):
+        n = max(8, int(round(n)))
+        cands = [m for m in range(max(8, n - 8), n + 9) if m % 2 == 0 and _smooth(m)]
+        return min(cands, key=lambda m: (abs(m - n), m))
+
+    sizes = [finest] + [smooth(finest / progression ** (0.5 * i)) for i in range(1, nlevels)]
+    npts = np.array([[n, n, n] for n in sizes], dtype=np.int32)
+    dh = np.array([cell / n for n in sizes])
+    dh_inv = np.array([np.linalg.inv(d) for d in dh])
+    hmax = np.array([max(np.linalg.norm(d, axis=1)) for d in dh])
+    cutoff = (math.pi / hmax) ** 2
+    return npts, dh, dh_inv, cutoff
+
+
+def _smooth(m):
+    for p in (2, 3, 5, 7):
+        while m % p == 0:
+            m //= p
+    return m == 1
+
+
+def gridlevel(zetp, cutoff, rel_cutoff):
+    """gaussian_gridlevel: coarsest level with cutoff + 1e-6 >= zetp*rel_cutoff
+    (level 1 if none)."""
+    need = np.abs(zetp) * rel_cutoff
+    level = np.ones(np.shape(zetp), dtype=np.int32)
+    for i, c in enumerate(cutoff):
+        level = np.where(c + 1e-6 >= need, i + 1, level)
+    return level
+
+
+# ---------------------------------------------------------------------------
+# task list
+# ---------------------------------------------------------------------------
+def build_task_list(cell, orthorhombic, positions, kinds, bases, npts, dh, dh_inv,
+                    cutoff, rel_cutoff, eps_rho, rng, flip_fraction=0.0,
+                    border_mask=None, npts_local=None, shift_local=None,
+                    border_width=None, meta=None):
+    """All tasks of all atom pairs (i <= j, with periodic images) that pass
+    the pgf screening.  Blocks belong to unordered atom pairs; a fraction of
+    blocks is stored with the atoms swapped to exercise the transposed
+    block layout."""
+    natoms = len(positions)
+    kinds = np.asarray(kinds, dtype=np.int32)
+    rp_kind = [pgf_radii(b) for b in bases]
+    kind_radius = np.array([r.max() for r in rp_kind])
+    heights = 1.0 / np.linalg.norm(np.linalg.inv(cell), axis=0)
+    rmax = 2.0 * kind_radius.max()
+    nimg = np.ceil(rmax / heights).astype(int)
+    images = np.array([(a, b, c) for a in range(-nimg[0], nimg[0] + 1)
+                       for b in range(-nimg[1], nimg[1] + 1)
+                       for c in range(-nimg[2], nimg[2] + 1)])
+    pairs_i, pairs_j, pairs_rab = [], [], []
+    ii, jj = np.triu_indices(natoms)
+    for T in images:
+        shift = T @ cell
+        positive = (T[0] > 0) or (T[0] == 0 and T[1] > 0) or (T[0] == 0 and T[1] == 0 and T[2] > 0)
+        zero = not T.any()
+        rab = positions[jj] + shift - positions[ii]
+        dab = np.linalg.norm(rab, axis=1)
+        ok = dab <= kind_radius[kinds[ii] - 1] + kind_radius[kinds[jj] - 1]
+        same = ii == jj
+        if zero:
+            pass
+        elif positive:
+            pass
+        else:
+            ok &= ~same  # on-site images: keep only one of +T / -T
+        pairs_i.append(ii[ok])
+        pairs_j.append(jj[ok])
+        pairs_rab.append(rab[ok])
+    pi = np.concatenate(pairs_i)
+    pj = np.concatenate(pairs_j)
+    prab = np.concatenate(pairs_rab)
+    pdab = np.linalg.norm(prab, axis=1)
+
+    # blocks: one per unordered atom pair present
+    pair_key = pi.astype(np.int64) * natoms + pj
+    ukey, pair_block = np.unique(pair_key, return_inverse=True)
+    nblocks = len(ukey)
+    bi = (ukey // natoms).astype(np.int64)
+    bj = (ukey % natoms).astype(np.int64)
+    nsgf = np.array([b.nsgf for b in bases])
+    sizes = nsgf[kinds[bi] - 1] * nsgf[kinds[bj] - 1]
+    block_offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    buffer_size = int(sizes.sum())
+    flip_block = rng.random(nblocks) < flip_fraction
+    flip_block[bi == bj] = False
+
+    out = {k: [] for k in ("level", "iatom", "jatom", "iset", "jset", "ipgf", "jpgf",
+                           "block", "radius", "rab")}
+    nk = len(bases)
+    for ka in range(1, nk + 1):
+        for kb in range(1, nk + 1):
+            sel = np.nonzero((kinds[pi] == ka) & (kinds[pj] == kb))[0]
+            if sel.size == 0:
+                continue
+            A, B = bases[ka - 1], bases[kb - 1]
+            ra_, rb_ = rp_kind[ka - 1], rp_kind[kb - 1]
+            for s1 in range(A.nset):
+                for s2 in range(B.nset):
+                    for p1 in range(int(A.npgf[s1])):
+                        for p2 in range(int(B.npgf[s2])):
+                            m = sel[ra_[s1, p1] + rb_[s2, p2] >= pdab[sel]]
+                            if m.size == 0:
+                                continue
+                            za, zb = A.zet[s1, p1], B.zet[s2, p2]
+                            rad = product_radius(int(A.lmax[s1]), int(B.lmax[s2]), za, zb,
+                                                 pdab[m], eps_rho)
+                            keep = rad > 0.0
+                            m, rad = m[keep], rad[keep]
+                            if m.size == 0:
+                                continue
+                            lev = gridlevel(np.full(m.size, za + zb), cutoff, rel_cutoff)
+                            out["level"].append(lev)
+                            out["iatom"].append(pi[m] + 1)
+                            out["jatom"].append(pj[m] + 1)
+                            out["iset"].append(np.full(m.size, s1 + 1))
+                            out["jset"].append(np.full(m.size, s2 + 1))
+                            out["ipgf"].append(np.full(m.size, p1 + 1))
+                            out["jpgf"].append(np.full(m.size, p2 + 1))
+                            out["block"].append(pair_block[m] + 1)
+                            out["radius"].append(rad)
+                            out["rab"].append(prab[m])
+    cat = {k: np.concatenate(v) if v else np.zeros((0, 3) if k == "rab" else 0)
+           for k, v in out.items()}
+    # swap atoms of flipped blocks: task (i,j,rab) -> (j,i,-rab)
+    fl = flip_block[cat["block"] - 1] if cat["block"].size else np.zeros(0, bool)
+    ia = np.where(fl, cat["jatom"], cat["iatom"])
+    ja = np.where(fl, cat["iatom"], cat["jatom"])
+    iset = np.where(fl, cat["jset"], cat["iset"])
+    jset = np.where(fl, cat["iset"], cat["jset"])
+    ipgf = np.where(fl, cat["jpgf"], cat["ipgf"])
+    jpgf = np.where(fl, cat["ipgf"], cat["jpgf"])
+    rab = np.where(fl[:, None], -cat["rab"], cat["rab"])
+    ntasks = len(ia)
+    perm = rng.permutation(ntasks)  # the API does not require any order
+    nlevels = len(npts)
+    bm = np.zeros(ntasks, dtype=np.int32) if border_mask is None else border_mask(ntasks)
+    args = TaskListArgs(
+        orthorhombic=orthorhombic, basis_sets=bases,
+        atom_positions=positions, atom_kinds=kinds, block_offsets=block_offsets,
+        buffer_size=buffer_size, level_list=cat["level"][perm],
+        iatom_list=ia[perm], jatom_list=ja[perm], iset_list=iset[perm],
+        jset_list=jset[perm], ipgf_list=ipgf[perm], jpgf_list=jpgf[perm],
+        border_mask_list=bm, block_num_list=cat["block"][perm],
+        radius_list=cat["radius"][perm], rab_list=rab[perm],
+        npts_global=npts,
+        npts_local=npts if npts_local is None else npts_local,
+        shift_local=np.zeros((nlevels, 3)) if shift_local is None else shift_local,
+        border_width=np.zeros((nlevels, 3)) if border_width is None else border_width,
+        dh=dh, dh_inv=dh_inv, meta=dict(meta or {}))
+    args.meta.update(nblocks=nblocks, block_i=bi + 1, block_j=bj + 1)
+    return args
+
+
+def random_density(args, rng, spd_dim_limit=4000):
+    """pab_blocks: symmetric random density matrix restricted to the blocks
+    of the list, in CP2K's block layout (column-major, rows = lower atom)."""
+    bases = args.basis_sets
+    kinds = args.atom_kinds
+    nsgf = np.array([b.nsgf for b in bases])
+    first = np.concatenate([[0], np.cumsum(nsgf[kinds - 1])])
+    dim = int(first[-1])
+    buf = np.zeros(args.buffer_size)
+    bi, bj = args.meta["block_i"], args.meta["block_j"]
+    if dim <= spd_dim_limit and args.meta.get("spd", False):
+        X = rng.normal(size=(dim, dim))
+        P = X @ X.T / dim
+        get = lambda i, j: P[first[i - 1]:first[i], first[j - 1]:first[j]]  # noqa: E731
+    else:
+        get = None
+    for b in range(len(args.block_offsets)):
+        i, j = int(bi[b]), int(bj[b])
+        ni, nj = nsgf[kinds[i - 1] - 1], nsgf[kinds[j - 1] - 1]
+        if get is not None:
+            blk = get(i, j)
+        else:
+            blk = rng.normal(size=(ni, nj))
+            if i == j:
+                blk = 0.5 * (blk + blk.T)
+        off = int(args.block_offsets[b])
+        buf[off:off + ni * nj] = blk.T.ravel()  # column-major, rows = atom i
+    return buf
+
+
+def random_potential(args, rng):
+    return [rng.normal(size=s) for s in args.grid_shapes()]
+
+
+# ---------------------------------------------------------------------------
+# configurations
+# ---------------------------------------------------------------------------
+def _water(rng, n, L, min_oo=2.5 * ANGSTROM):
+    roh = 0.9572 * ANGSTROM
+    ang = math.radians(104.52)
+    local = np.array([[0.0, 0.0, 0.0],
+                      [roh, 0.0, 0.0],
+                      [roh * math.cos(ang), roh * math.sin(ang), 0.0]])
+    centers = []
+    tries = 0
+    while len(centers) < n:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("could not place molecules")
+        c = rng.uniform(0, L, 3)
+        ok = True
+        for o in centers:
+            d = c - o
+            d -= L * np.round(d / L)
+            if np.linalg.norm(d) < min_oo:
+                ok = False
+                break
+        if ok:
+            centers.append(c)
+    pos, kinds = [], []
+    for c in centers:
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        for a, k in zip(local @ R.T + c, (1, 2, 2)):
+            pos.append(np.mod(a, L))
+            kinds.append(k)
+    return np.array(pos), np.array(kinds, dtype=np.int32)
+
+
+def config(name, seed=0, **kw):
+    """Returns (TaskListArgs, pab_blocks, [V per level])."""
+    rng = np.random.default_rng(seed)
+    if name == "ref32" or name.startswith("ref"):
+        natoms = 32 if name == "ref32" else int(name[3:])
+        L = 10.0 * ANGSTROM
+        cell = cubic_cell(L)
+        npts, dh, dh_inv, cutoff = grid_levels(cell, kw.get("n", 64), 1)
+        bases = [make_basis(rng, [(0, 0, 1, 1), (1, 1, 1, 1), (2, 2, 1, 1)], 0.1, 20.0,
+                            primitive=True) for _ in range(4)]
+        pos = rng.uniform(0, L, (natoms, 3))
+        kinds = (np.arange(natoms) % 4) + 1
+        args = build_task_list(cell, True, pos, kinds, bases, npts, dh, dh_inv, cutoff,
+                               1.0e9, 1.0e-10, rng, flip_fraction=kw.get("flip", 0.0),
+                               meta=dict(config=name))
+    elif name.startswith("h2o"):
+        nmol = 64 if name == "h2o64" else int(name[3:])
+        L = 12.4 * ANGSTROM * (nmol / 64.0) ** (1.0 / 3.0)
+        cell = cubic_cell(L)
+        finest = kw.get("n", 128 if nmol == 64 else _even_smooth(128 * L / (12.4 * ANGSTROM)))
+        npts, dh, dh_inv, cutoff = grid_levels(cell, finest, kw.get("nlevels", 4))
+        o = make_basis(rng, [(0, 2, 6, 2)], 0.05, 50.0)
+        o.nsgf_set[:] = o.nsgf_set  # 2s 2p 2d
+        h = make_basis(rng, [(0, 1, 5, 2)], 0.05, 50.0)
+        pos, kinds = _water(rng, nmol, L)
+        args = build_task_list(cell, True, pos, kinds, [o, h], npts, dh, dh_inv, cutoff,
+                               kw.get("rel_cutoff", 30.0), kw.get("eps", 1.0e-10), rng,
+                               flip_fraction=kw.get("flip", 0.3),
+                               meta=dict(config=name, spd=True))
+    elif name.startswith("tri"):
+        natoms = 1024 if name == "tri1024" else int(name[3:])
+        vol = natoms / 0.1 * ANGSTROM ** 3
+        cell = triclinic_cell(rng, vol)
+        scale = (natoms / 1024.0) ** (1.0 / 3.0)
+        finest = kw.get("n", 200 if natoms == 1024 else _even_smooth(200 * scale))
+        npts, dh, dh_inv, cutoff = grid_levels(cell, finest, kw.get("nlevels", 4))
+        a = make_basis(rng, [(0, 1, 5, 3), (2, 2, 2, 2), (3, 3, 1, 1)], 0.03, 100.0)
+        b = make_basis(rng, [(0, 1, 4, 2), (2, 2, 1, 1)], 0.03, 100.0)
+        frac = rng.uniform(0, 1, (natoms, 3))
+        pos = frac @ cell
+        kinds = np.where(np.arange(natoms) % 3 == 0, 1, 2).astype(np.int32)
+        args = build_task_list(cell, False, pos, kinds, [a, b], npts, dh, dh_inv, cutoff,
+                               kw.get("rel_cutoff", 40.0), kw.get("eps", 1.0e-10), rng,
+                               flip_fraction=kw.get("flip", 0.3), meta=dict(config=name))
+    elif name.startswith("hl"):
+        natoms = 2048 if name == "hl2048" else int(name[2:])
+        L = (natoms / 0.1) ** (1.0 / 3.0) * ANGSTROM
+        cell = cubic_cell(L)
+        scale = (natoms / 2048.0) ** (1.0 / 3.0)
+        finest = kw.get("n", 256 if natoms == 2048 else _even_smooth(256 * scale))
+        npts, dh, dh_inv, cutoff = grid_levels(cell, finest, kw.get("nlevels", 5))
+        a = make_basis(rng, [(0, 2, 4, 2), (3, 4, 2, 1)], 0.02, 200.0)
+        b = make_basis(rng, [(0, 1, 3, 2), (2, 2, 1, 1)], 0.02, 200.0)
+        pos = rng.uniform(0, L, (natoms, 3))
+        kinds = np.where(np.arange(natoms) % 2 == 0, 1, 2).astype(np.int32)
+        args = build_task_list(cell, True, pos, kinds, [a, b], npts, dh, dh_inv, cutoff,
+                               kw.get("rel_cutoff", 40.0), kw.get("eps", 1.0e-10), rng,
+                               flip_fraction=kw.get("flip", 0.3), meta=dict(config=name))
+    else:
+        raise ValueError(f"unknown configuration {name}")
+    pab = random_density(args, rng)
+    V = random_potential(args, rng)
+    return args, pab, V
+
+
+def _even_smooth(n):
+    n = max(16, int(round(n)))
+    while not (n % 2 == 0 and _smooth(n)):
+        n += 1
+    return n
+
+
+def describe(args):
+    lv = np.bincount(args.level_list, minlength=args.nlevels + 1)[1:]
+    return (f"tasks={args.ntasks} blocks={args.nblocks} atoms={args.natoms} "
+            f"levels={[tuple(int(x) for x in n) for n in args.npts_local]} "
+            f"tasks/level={lv.tolist()} pab={args.buffer_size}")
+
+
+if __name__ == "__main__":
+    import time
+    for nm in sys.argv[1:] or ["ref32"]:
+        t0 = time.time()
+        a, _, _ = config(nm)
+        print(nm, describe(a), f"({time.time() - t0:.1f} s)")

@@ -1,0 +1,109 @@
+/*
+ * grid_mi355x.h -- backend-level drop-in symbols and MI355X extensions.
+ *
+ * Part 1 replaces src/grid/gpu/grid_gpu_task_list.h:25-59, so the library can
+ * stand in for the .cu files of src/grid/gpu inside an unmodified CP2K build whose
+ * dispatcher (src/grid/grid_task_list.c) stays in place.
+ *
+ * Part 2 has no counterpart in the reference: precision selection, HBM-
+ * resident entry points (no host copies; for callers that keep grids and
+ * matrices on the device, and for measuring the kernels without PCIe), the
+ * stream the library launches on, and per-kernel timing.
+ */
+#ifndef GRID_MI355X_H
+#define GRID_MI355X_H
+
+#include <stdbool.h>
+
+#include "grid_task_list.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Part 1: src/grid/gpu/grid_gpu_task_list.h:25-59 -------------------- */
+typedef void grid_gpu_task_list;
+
+void grid_gpu_create_task_list(
+    const bool ortho, const int ntasks, const int nlevels, const int natoms,
+    const int nkinds, const int nblocks, const int *block_offsets,
+    const double *atom_positions, const int *atom_kinds,
+    const grid_basis_set **basis_sets, const int *level_list,
+    const int *iatom_list, const int *jatom_list, const int *iset_list,
+    const int *jset_list, const int *ipgf_list, const int *jpgf_list,
+    const int *border_mask_list, const int *block_num_list,
+    const double *radius_list, const double *rab_list, const int *npts_global,
+    const int *npts_local, const int *shift_local, const int *border_width,
+    const double *dh, const double *dh_inv, grid_gpu_task_list **ptr);
+
+void grid_gpu_free_task_list(grid_gpu_task_list *ptr);
+
+void grid_gpu_collocate_task_list(const grid_gpu_task_list *ptr,
+                                  const enum grid_func func, const int nlevels,
+                                  const offload_buffer *pab_blocks,
+                                  offload_buffer **grids);
+
+void grid_gpu_integrate_task_list(const grid_gpu_task_list *ptr,
+                                  const bool compute_tau, const int nlevels,
+                                  const offload_buffer *pab_blocks,
+                                  const offload_buffer **grids,
+                                  offload_buffer *hab_blocks, double *forces,
+                                  double *virial);
+
+/* ---- Part 2: extensions (no reference counterpart) ----------------------- */
+
+/* Arithmetic of the per-point stage.  FP64: everything in double (matches
+ * the reference to summation order).  FP32_MIXED: the Gaussian x polynomial
+ * evaluation at each grid point runs in float; coefficient transforms,
+ * factor tables, index math and every accumulation stay double.  Default
+ * FP64, or the value of environment variable GRID_MI355X_PRECISION (0/1). */
+enum grid_mi355x_precision {
+  GRID_MI355X_FP64 = 0,
+  GRID_MI355X_FP32_MIXED = 1,
+};
+void grid_mi355x_set_precision(int precision);
+int grid_mi355x_get_precision(void);
+
+/* HBM-resident collocate: pab_blocks and grids[level] are device pointers;
+ * work is enqueued on the library stream and the call returns without
+ * waiting (see grid_mi355x_synchronize). */
+void grid_mi355x_collocate_task_list_dev(const grid_task_list *task_list,
+                                         const enum grid_func func,
+                                         const int nlevels,
+                                         const double *pab_blocks,
+                                         double *const *grids);
+
+/* HBM-resident integrate: hab_blocks (hab_length doubles) is overwritten;
+ * forces [natoms][3] and virial [3][3] are device pointers or NULL. */
+void grid_mi355x_integrate_task_list_dev(
+    const grid_task_list *task_list, const bool compute_tau, const int natoms,
+    const int nlevels, const double *pab_blocks, const double *const *grids,
+    double *hab_blocks, const long hab_length, double *forces, double *virial);
+
+/* Stream all library work is launched on (a hipStream_t; NULL restores the
+ * library's own stream). */
+void grid_mi355x_set_stream(void *hip_stream);
+void *grid_mi355x_get_stream(void);
+void grid_mi355x_synchronize(void);
+
+/* Per-kernel timing with HIP events on the launch stream.  Kernel ids:
+ * 0 collocate points, 1 integrate points, 2 decontract pab, 3 contract hab,
+ * 4 zero/fill, 5 force/virial reduction. */
+void grid_mi355x_set_timing(const bool enable);
+void grid_mi355x_reset_timing(void);
+/* Returns total milliseconds and number of launches of kernel id. */
+double grid_mi355x_kernel_time_ms(const int kernel_id, long *launches);
+
+/* Number of (task, grid point) pairs the list visits, counted on the device
+ * with the same membership rules as collocate/integrate. */
+long long grid_mi355x_task_points(const grid_task_list *task_list);
+
+/* Number of tasks that were assigned to the orthorhombic / general kernels. */
+void grid_mi355x_task_counts(const grid_task_list *task_list, long *ortho,
+                             long *general, long *skipped);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
