@@ -577,6 +577,18 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->points = -1;
 }
 
+// Largest dynamic LDS allocation per workgroup on the current device.
+int lds_limit() {
+  static int limit = 0;
+  if (!limit) {
+    int dev = 0, v = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    limit = v > 0 ? v : 65536;
+  }
+  return limit;
+}
+
 // LDS plan for one launch; ld are the l-range shifts of cab.
 LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
   LdsPlan p{};
@@ -593,15 +605,23 @@ LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
     p.rowcap = rowcap;
     p.plcap = plcap;
     int o = 0;
-    p.o_cab = o; o += n_cab;
-    p.o_alpha = o; o += n_alpha;
     p.o_cxyz = o; o += n_cube;
     p.o_cijk = o; o += tl->has_general ? n_cube : 0;
-    p.o_tab = o; o += tl->has_ortho ? 3 * p.np * p.ext : 0;
-    p.o_cxy = o; o += plcap * p.np * p.np;
-    p.o_cxrow = o; o += rowcap * p.np;
     p.o_red = o; o += 4 * kTaskOutStride;
-    p.n_doubles = o;
+    // Setup/finish phase (cab, alpha) and walking phase (tables, plane and
+    // row coefficients) never overlap in time and share one region.
+    const int u = o;
+    p.o_cab = u;
+    p.o_alpha = u + n_cab;
+    const int phase_a = n_cab + n_alpha;
+    p.o_tab = u;
+    p.o_cxy = p.o_tab + (tl->has_ortho ? 3 * p.np * p.ext : 0);
+    p.o_cxrow = p.o_cxy + plcap * p.np * p.np;
+    p.o_stage = p.o_cxrow + rowcap * p.np;
+    const int lpb = lpb_bucket(lp);
+    const int stage = (integrate && lpb <= 8) ? 64 * ((lpb + 1) * (lpb + 2) / 2 + lpb + 1) : 0;
+    const int phase_b = p.o_stage + stage - u;
+    p.n_doubles = u + std::max(phase_a, phase_b);
     int i = 0;
     p.i_map = i; i += 3 * p.mapn;
     p.i_plane = i; i += 3 * (plcap + 1);
@@ -611,12 +631,13 @@ LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
     p.bytes = 8 * p.n_doubles + 4 * p.n_ints;
   };
   const int planes = std::max(1, tl->max_planes);
+  const int limit = lds_limit();
   layout(std::max(need_rows, 256), std::min(64, planes));
   if (p.bytes > 48 * 1024) layout(std::max(need_rows, 128), std::min(16, planes));
   if (p.bytes > 64 * 1024) layout(need_rows, std::min(4, planes));
-  if (p.bytes > 64 * 1024)
+  if (p.bytes > limit)
     fatal("task list needs %d bytes of LDS per task (lp=%d, cube half-width %d); "
-          "the limit is 65536", p.bytes, lp, tl->max_cmax);
+          "the device limit is %d", p.bytes, lp, tl->max_cmax, limit);
   (void)integrate;
   return p;
 }
@@ -1145,6 +1166,30 @@ void grid_mi355x_integrate_task_list_dev(const grid_task_list *task_list, const 
     fatal("hab_blocks holds %ld doubles but the task list addresses %lld", hab_length,
           tl->block_end);
   run_integrate(tl, compute_tau, pab_blocks, grids, hab_blocks, hab_length, forces, virial, s);
+}
+
+void *grid_mi355x_device_malloc(const size_t bytes) {
+  ensure_device();
+  void *p = nullptr;
+  HIP_CHECK(hipMalloc(&p, bytes ? bytes : 8));
+  return p;
+}
+
+void grid_mi355x_device_free(void *ptr) {
+  if (ptr) HIP_CHECK(hipFree(ptr));
+}
+
+void grid_mi355x_memcpy(void *dst, const void *src, const size_t bytes, const int kind) {
+  hipMemcpyKind k;
+  switch (kind) {
+    case 1: k = hipMemcpyHostToDevice; break;
+    case 2: k = hipMemcpyDeviceToHost; break;
+    case 3: k = hipMemcpyDeviceToDevice; break;
+    default: fatal("unknown copy kind %d", kind);
+  }
+  const hipStream_t s = stream();
+  HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, s));
+  sync_stream(s);
 }
 
 void grid_mi355x_set_stream(void *hip_stream) {

@@ -75,11 +75,17 @@ struct LdsPlan {
   int mapn;      // ints per map array
   int rowcap, plcap;
   int o_cab, o_alpha, o_cxyz, o_cijk, o_tab, o_cxy, o_cxrow, o_red;  // doubles
+  int o_stage;   // per-column moment staging of the orthorhombic integrate
   int n_doubles;
   int i_map, i_row, i_plane, i_misc;  // ints, after the doubles
   int n_ints;
   int bytes;
 };
+
+// Register-array bound of the point kernels for a maximum lp.
+inline int lpb_bucket(int lp) {
+  return lp <= 2 ? 2 : lp <= 4 ? 4 : lp <= 6 ? 6 : lp <= 8 ? 8 : lp <= 12 ? 12 : 16;
+}
 
 struct PointParams {
   const Task *tasks;

@@ -526,7 +526,27 @@ __device__ __forceinline__ void stage_task(const PointParams &P, Task &sT, Level
 // ---------------------------------------------------------------------------
 // Collocate: one workgroup per task
 // ---------------------------------------------------------------------------
+// Lanes of a workgroup as slots of `width` lanes, one x column per lane:
+// slot s handles rows s, s + nslots, ...; lane q of the slot handles the
+// cube column lo[0] + pass*width + q.
+struct XSlots {
+  int width, slot, nslots, q, npass;
+};
+
 template <int NW>
+__device__ __forceinline__ XSlots x_slots(const Task &T, int tid) {
+  const int extent = 2 - 2 * T.lo[0];  // columns lo[0] .. 1 - lo[0]
+  XSlots X;
+  X.width = extent <= 16 ? 16 : (extent <= 32 ? 32 : 64);
+  const int per_wave = 64 / X.width;
+  X.slot = (tid >> 6) * per_wave + (tid & 63) / X.width;
+  X.nslots = per_wave * NW;
+  X.q = (tid & 63) % X.width;
+  X.npass = (extent + X.width - 1) / X.width;
+  return X;
+}
+
+template <int NW, int LPB>
 __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
   extern __shared__ double lds[];
   __shared__ Task sT;
@@ -599,6 +619,8 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
               atomicAdd(&cxyz[(lz * NP + ly) * NP + lx], cv * (ax[lx] * ay[ly] * az[lz] * pref));
       }
     }
+    // cab/alpha share LDS with the per-axis tables and chunk arrays
+    __syncthreads();
     if (ortho) {
       build_ortho_tables(T, lay, lp, pl, tab, map, tid, nthr);
       __syncthreads();
@@ -673,6 +695,34 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
       cxrow[e] = s;
     }
     __syncthreads();
+    if (ortho) {
+      // ---- points: each lane owns one x column of the cube ----
+      const XSlots X = x_slots<NW>(T, tid);
+      for (int pass = 0; pass < X.npass; pass++) {
+        const int i = T.lo[0] + pass * X.width + X.q;
+        const bool iv = i <= 1 - T.lo[0];
+        double px[LPB + 1];
+#pragma unroll
+        for (int l = 0; l <= LPB; l++)
+          px[l] = (iv && l <= lp) ? tab[(0 * NP + l) * EXT + i + cmax] : 0.0;
+        const long long ig = iv ? mapx[i + cmax] : 0;
+        for (int r = X.slot; r < nrows; r += X.nslots) {
+          const int i0 = C.ri[r];
+          if (!iv || i < i0 || i >= i0 + (C.roff[r + 1] - C.roff[r])) continue;
+          const double *cr = &cxrow[r * NP];
+          double v = 0.0;
+#pragma unroll
+          for (int l = 0; l <= LPB; l++)
+            if (l <= lp) v = fma(cr[l], px[l], v);
+          const int j = C.rj[r], k = C.pk[C.rp[r]];
+          const long long gidx = ((long long)mapz[k + cmax] * ny + mapy[j + cmax]) * nx + ig;
+          if (gidx < L.npts) atomicAdd(&L.grid[gidx], v);
+        }
+      }
+      __syncthreads();
+      kbeg += npl;
+      continue;
+    }
     // ---- points: consecutive lanes take consecutive points of the chunk ----
     for (int base = 0; base < npts; base += nthr) {
       const int q = base + tid;
@@ -856,6 +906,84 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   for (int kbeg = kbeg0; kbeg <= kend;) {
     const int npl = build_chunk<NW>(T, lay, ortho, kbeg, pl, C, map, tid, nthr);
     const int nrows = C.misc[1];
+    if (ortho && LPB <= 8) {
+      // ---- each lane owns one x column: it accumulates the (y, z) moments
+      //      Q[ly][lz] = sum_rows V(i,j,k) py_ly(j) pz_lz(k) of its column in
+      //      registers; columns are then contracted with px into cxyz ----
+      constexpr int NQ = (LPB + 1) * (LPB + 2) / 2;
+      constexpr int SW = NQ + LPB + 1;  // staging words per column
+      const XSlots X = x_slots<NW>(T, tid);
+      double *stage = lds + pl.o_stage;
+      for (int pass = 0; pass < X.npass; pass++) {
+        for (int e = tid; e < X.width * SW; e += nthr) stage[e] = 0.0;
+        __syncthreads();
+        const int i = T.lo[0] + pass * X.width + X.q;
+        const bool iv = i <= 1 - T.lo[0];
+        double px[LPB + 1], R[LPB + 1], Q[NQ];
+#pragma unroll
+        for (int l = 0; l <= LPB; l++) {
+          px[l] = (iv && l <= lp) ? tab[(0 * NP + l) * EXT + i + cmax] : 0.0;
+          R[l] = 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < NQ; e++) Q[e] = 0.0;
+        const long long ig = iv ? mapx[i + cmax] : 0;
+        int cur = -1;
+        auto flush = [&](int plane) {
+          const int k = C.pk[plane];
+          int e = 0;
+#pragma unroll
+          for (int ly = 0; ly <= LPB; ly++) {
+#pragma unroll
+            for (int lz = 0; lz <= LPB - ly; lz++) {
+              if (ly + lz <= lp) Q[e] = fma(R[ly], tab[(2 * NP + lz) * EXT + k + cmax], Q[e]);
+              e++;
+            }
+          }
+#pragma unroll
+          for (int l = 0; l <= LPB; l++) R[l] = 0.0;
+        };
+        for (int r = X.slot; r < nrows; r += X.nslots) {
+          const int p = C.rp[r];
+          if (p != cur) {
+            if (cur >= 0) flush(cur);
+            cur = p;
+          }
+          const int i0 = C.ri[r];
+          if (!iv || i < i0 || i >= i0 + (C.roff[r + 1] - C.roff[r])) continue;
+          const int j = C.rj[r], k = C.pk[p];
+          const long long gidx = ((long long)mapz[k + cmax] * ny + mapy[j + cmax]) * nx + ig;
+          const double g = (gidx < L.npts) ? grid[gidx] : 0.0;
+#pragma unroll
+          for (int ly = 0; ly <= LPB; ly++)
+            if (ly <= lp) R[ly] = fma(g, tab[(1 * NP + ly) * EXT + j + cmax], R[ly]);
+        }
+        if (cur >= 0) flush(cur);
+        if (iv) {
+          double *sq = &stage[X.q * SW];
+#pragma unroll
+          for (int e = 0; e < NQ; e++)
+            if (Q[e] != 0.0) atomicAdd(&sq[e], Q[e]);
+          if (X.slot == 0) {
+#pragma unroll
+            for (int l = 0; l <= LPB; l++) sq[NQ + l] = px[l];
+          }
+        }
+        __syncthreads();
+        // cxyz[lz][ly][lx] += sum over columns of px[lx] * Q[ly][lz]
+        for (int e = tid; e < NP * NP * NP; e += nthr) {
+          const int lz = e / (NP * NP), ly = (e / NP) % NP, lx = e % NP;
+          if (lz + ly + lx > lp) continue;
+          const int qi = ly * (LPB + 1) - (ly * (ly - 1)) / 2 + lz;
+          double s = 0.0;
+          for (int c = 0; c < X.width; c++) s = fma(stage[c * SW + NQ + lx], stage[c * SW + qi], s);
+          cxyz[e] += s;
+        }
+        __syncthreads();
+      }
+      kbeg += npl;
+      continue;
+    }
     // ---- rows: each lane integrates whole rows into registers ----
     for (int r = tid; r < nrows; r += nthr) {
       double acc[LPB + 1];
@@ -1139,47 +1267,69 @@ __global__ __launch_bounds__(256) void reduce_virial_kernel(const double *task_o
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from grid_api.cpp)
 // ---------------------------------------------------------------------------
-static int lpb_bucket(int lp) {
-  if (lp <= 2) return 2;
-  if (lp <= 4) return 4;
-  if (lp <= 6) return 6;
-  if (lp <= 8) return 8;
-  if (lp <= 12) return 12;
-  return 16;
+
+// Dynamic LDS above 64 KiB has to be requested per kernel.
+template <typename K>
+static hipError_t allow_lds(K kernel, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-hipError_t launch_collocate(const PointParams &P, int nwaves, hipStream_t s) {
-  const size_t lds = P.plan.bytes;
-  const dim3 grid(P.ntasks), block(64 * nwaves);
-  switch (nwaves) {
-    case 1: hipLaunchKernelGGL(collocate_kernel<1>, grid, block, lds, s, P); break;
-    case 2: hipLaunchKernelGGL(collocate_kernel<2>, grid, block, lds, s, P); break;
-    default: hipLaunchKernelGGL(collocate_kernel<4>, grid, block, lds, s, P); break;
-  }
+template <typename K>
+static hipError_t launch_task_kernel(K kernel, const PointParams &P, int nthreads, size_t lds,
+                                     hipStream_t s) {
+  const hipError_t e = allow_lds(kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kernel, dim3(P.ntasks), dim3(nthreads), lds, s, P);
   return hipGetLastError();
 }
 
 template <int NW>
-static void launch_integrate_nw(const PointParams &P, int lp, size_t lds, hipStream_t s) {
-  const dim3 grid(P.ntasks), block(64 * NW);
+static hipError_t launch_collocate_nw(const PointParams &P, int lp, size_t lds, hipStream_t s) {
+  const int n = 64 * NW;
   switch (lpb_bucket(lp)) {
-    case 2: hipLaunchKernelGGL((integrate_kernel<NW, 2>), grid, block, lds, s, P); break;
-    case 4: hipLaunchKernelGGL((integrate_kernel<NW, 4>), grid, block, lds, s, P); break;
-    case 6: hipLaunchKernelGGL((integrate_kernel<NW, 6>), grid, block, lds, s, P); break;
-    case 8: hipLaunchKernelGGL((integrate_kernel<NW, 8>), grid, block, lds, s, P); break;
-    case 12: hipLaunchKernelGGL((integrate_kernel<NW, 12>), grid, block, lds, s, P); break;
-    default: hipLaunchKernelGGL((integrate_kernel<NW, 16>), grid, block, lds, s, P); break;
+    case 2: return launch_task_kernel(collocate_kernel<NW, 2>, P, n, lds, s);
+    case 4: return launch_task_kernel(collocate_kernel<NW, 4>, P, n, lds, s);
+    case 6: return launch_task_kernel(collocate_kernel<NW, 6>, P, n, lds, s);
+    case 8: return launch_task_kernel(collocate_kernel<NW, 8>, P, n, lds, s);
+    case 12: return launch_task_kernel(collocate_kernel<NW, 12>, P, n, lds, s);
+    default: return launch_task_kernel(collocate_kernel<NW, 16>, P, n, lds, s);
+  }
+}
+
+hipError_t launch_collocate(const PointParams &P, int nwaves, hipStream_t s) {
+  const size_t lds = P.plan.bytes;
+  const int lp = P.plan.np - 1;
+  if (lp > 16) return hipErrorInvalidValue;
+  switch (nwaves) {
+    case 1: return launch_collocate_nw<1>(P, lp, lds, s);
+    case 2: return launch_collocate_nw<2>(P, lp, lds, s);
+    default: return launch_collocate_nw<4>(P, lp, lds, s);
+  }
+}
+
+template <int NW>
+static hipError_t launch_integrate_nw(const PointParams &P, int lp, size_t lds, hipStream_t s) {
+  const int n = 64 * NW;
+  switch (lpb_bucket(lp)) {
+    case 2: return launch_task_kernel(integrate_kernel<NW, 2>, P, n, lds, s);
+    case 4: return launch_task_kernel(integrate_kernel<NW, 4>, P, n, lds, s);
+    case 6: return launch_task_kernel(integrate_kernel<NW, 6>, P, n, lds, s);
+    case 8: return launch_task_kernel(integrate_kernel<NW, 8>, P, n, lds, s);
+    case 12: return launch_task_kernel(integrate_kernel<NW, 12>, P, n, lds, s);
+    default: return launch_task_kernel(integrate_kernel<NW, 16>, P, n, lds, s);
   }
 }
 
 hipError_t launch_integrate(const PointParams &P, int max_lp, int nwaves, hipStream_t s) {
   const size_t lds = P.plan.bytes;
+  if (max_lp > 16) return hipErrorInvalidValue;
   switch (nwaves) {
-    case 1: launch_integrate_nw<1>(P, max_lp, lds, s); break;
-    case 2: launch_integrate_nw<2>(P, max_lp, lds, s); break;
-    default: launch_integrate_nw<4>(P, max_lp, lds, s); break;
+    case 1: return launch_integrate_nw<1>(P, max_lp, lds, s);
+    case 2: return launch_integrate_nw<2>(P, max_lp, lds, s);
+    default: return launch_integrate_nw<4>(P, max_lp, lds, s);
   }
-  return hipGetLastError();
 }
 
 hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,

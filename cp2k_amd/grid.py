@@ -28,7 +28,8 @@ EXPORTS = [
     "grid_gpu_free_task_list", "grid_gpu_collocate_task_list",
     "grid_gpu_integrate_task_list", "grid_mi355x_set_precision",
     "grid_mi355x_get_precision", "grid_mi355x_collocate_task_list_dev",
-    "grid_mi355x_integrate_task_list_dev", "grid_mi355x_set_stream",
+    "grid_mi355x_integrate_task_list_dev", "grid_mi355x_device_malloc",
+    "grid_mi355x_device_free", "grid_mi355x_memcpy", "grid_mi355x_set_stream",
     "grid_mi355x_get_stream", "grid_mi355x_synchronize",
     "grid_mi355x_set_timing", "grid_mi355x_reset_timing",
     "grid_mi355x_kernel_time_ms", "grid_mi355x_task_points",
@@ -90,6 +91,10 @@ def lib():
         L.grid_mi355x_get_stream.restype = C.c_void_p
         L.grid_mi355x_set_stream.argtypes = [C.c_void_p]
         L.grid_mi355x_set_timing.argtypes = [C.c_bool]
+        L.grid_mi355x_device_malloc.restype = C.c_void_p
+        L.grid_mi355x_device_malloc.argtypes = [C.c_size_t]
+        L.grid_mi355x_device_free.argtypes = [C.c_void_p]
+        L.grid_mi355x_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _lib = L
     return _lib
 
@@ -139,6 +144,46 @@ class OffloadBuf:
         if getattr(self, "h", None):
             lib().offload_free_buffer(self.h)
             self.h = None
+
+
+class DeviceArray:
+    """A float64 array in HBM allocated through the library (no torch in the
+    process: torch ships its own HIP runtime, which cannot share a process
+    with the system ROCm runtime this library links)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in np.atleast_1d(shape)) if np.ndim(shape) else (int(shape),)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = 8 * self.size
+        self.ptr = lib().grid_mi355x_device_malloc(C.c_size_t(self.nbytes))
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        lib().grid_mi355x_memcpy(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p),
+                                 C.c_size_t(d.nbytes), C.c_int(1))
+        return d
+
+    def numpy(self):
+        a = np.empty(self.shape)
+        lib().grid_mi355x_memcpy(a.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                 C.c_size_t(self.nbytes), C.c_int(2))
+        return a
+
+    def data_ptr(self):
+        return self.ptr
+
+    def numel(self):
+        return self.size
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().grid_mi355x_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
 
 
 def set_precision(p):
@@ -218,9 +263,9 @@ class GridTaskList:
             hab.h, _ptr(f), _ptr(v))
         return hab.host().copy(), f, v
 
-    # ---- HBM-resident API (extension) on torch tensors ----
+    # ---- HBM-resident API (extension) on DeviceArray buffers ----
     def collocate_dev(self, func, pab, grids):
-        """pab: float64 device tensor; grids: list of float64 device tensors."""
+        """pab, grids[level]: DeviceArray (anything with data_ptr())."""
         a = self.args
         gp = (C.c_void_p * a.nlevels)(*[g.data_ptr() for g in grids])
         lib().grid_mi355x_collocate_task_list_dev(

@@ -429,8 +429,18 @@ def _water(rng, n, L, min_oo=2.5 * ANGSTROM):
     return np.array(pos), np.array(kinds, dtype=np.int32)
 
 
+def _keep(pos, kinds, kw, per=1):
+    """Optionally keep only the first kw['keep'] atoms (molecules of `per`
+    atoms) -- the cell and grids of the full configuration stay as they are."""
+    k = kw.get("keep")
+    if k is None:
+        return pos, kinds
+    return pos[:k * per], kinds[:k * per]
+
+
 def config(name, seed=0, **kw):
-    """Returns (TaskListArgs, pab_blocks, [V per level])."""
+    """Returns (TaskListArgs, pab_blocks, [V per level]).  keep=N keeps N atoms
+    (N molecules for h2o) of the configuration in its full-size cell."""
     rng = np.random.default_rng(seed)
     if name == "ref32" or name.startswith("ref"):
         natoms = 32 if name == "ref32" else int(name[3:])
@@ -441,6 +451,7 @@ def config(name, seed=0, **kw):
                             primitive=True) for _ in range(4)]
         pos = rng.uniform(0, L, (natoms, 3))
         kinds = (np.arange(natoms) % 4) + 1
+        pos, kinds = _keep(pos, kinds, kw)
         args = build_task_list(cell, True, pos, kinds, bases, npts, dh, dh_inv, cutoff,
                                1.0e9, 1.0e-10, rng, flip_fraction=kw.get("flip", 0.0),
                                meta=dict(config=name))
@@ -454,6 +465,7 @@ def config(name, seed=0, **kw):
         o.nsgf_set[:] = o.nsgf_set  # 2s 2p 2d
         h = make_basis(rng, [(0, 1, 5, 2)], 0.05, 50.0)
         pos, kinds = _water(rng, nmol, L)
+        pos, kinds = _keep(pos, kinds, kw, per=3)
         args = build_task_list(cell, True, pos, kinds, [o, h], npts, dh, dh_inv, cutoff,
                                kw.get("rel_cutoff", 30.0), kw.get("eps", 1.0e-10), rng,
                                flip_fraction=kw.get("flip", 0.3),
@@ -470,6 +482,7 @@ def config(name, seed=0, **kw):
         frac = rng.uniform(0, 1, (natoms, 3))
         pos = frac @ cell
         kinds = np.where(np.arange(natoms) % 3 == 0, 1, 2).astype(np.int32)
+        pos, kinds = _keep(pos, kinds, kw)
         args = build_task_list(cell, False, pos, kinds, [a, b], npts, dh, dh_inv, cutoff,
                                kw.get("rel_cutoff", 40.0), kw.get("eps", 1.0e-10), rng,
                                flip_fraction=kw.get("flip", 0.3), meta=dict(config=name))
@@ -484,6 +497,7 @@ def config(name, seed=0, **kw):
         b = make_basis(rng, [(0, 1, 3, 2), (2, 2, 1, 1)], 0.02, 200.0)
         pos = rng.uniform(0, L, (natoms, 3))
         kinds = np.where(np.arange(natoms) % 2 == 0, 1, 2).astype(np.int32)
+        pos, kinds = _keep(pos, kinds, kw)
         args = build_task_list(cell, True, pos, kinds, [a, b], npts, dh, dh_inv, cutoff,
                                kw.get("rel_cutoff", 40.0), kw.get("eps", 1.0e-10), rng,
                                flip_fraction=kw.get("flip", 0.3), meta=dict(config=name))

@@ -80,6 +80,13 @@ void grid_mi355x_integrate_task_list_dev(
     const int nlevels, const double *pab_blocks, const double *const *grids,
     double *hab_blocks, const long hab_length, double *forces, double *virial);
 
+/* HBM buffers for callers without their own device allocator (tests,
+ * benchmarks).  Copies are synchronous; kind 1 host->device, 2 device->host,
+ * 3 device->device.  Aborts on failure like every other entry point. */
+void *grid_mi355x_device_malloc(const size_t bytes);
+void grid_mi355x_device_free(void *ptr);
+void grid_mi355x_memcpy(void *dst, const void *src, const size_t bytes, const int kind);
+
 /* Stream all library work is launched on (a hipStream_t; NULL restores the
  * library's own stream). */
 void grid_mi355x_set_stream(void *hip_stream);

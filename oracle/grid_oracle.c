@@ -1055,6 +1055,13 @@ static int thread_id(void) {
 
 void oracle_collocate_task_list(const oracle_task_list *tl, int func, int nlevels,
                                 const double *pab_blocks, double *const *grids) {
+  oracle_collocate_task_list_with(tl, func, nlevels, pab_blocks, grids,
+                                  oracle_collocate_pgf_product);
+}
+
+void oracle_collocate_task_list_with(const oracle_task_list *tl, int func, int nlevels,
+                                     const double *pab_blocks, double *const *grids,
+                                     oracle_collocate_fn kernel) {
   assert(nlevels == tl->nlevels);
   int ld[4];
   if (!func_ldiffs(func, ld)) abort();
@@ -1089,7 +1096,7 @@ void oracle_collocate_task_list(const oracle_task_list *tl, int func, int nlevel
             old_jset = js;
           }
           const int nca = ncart(ib->lmax[is]), ncb = ncart(jb->lmax[js]);
-          oracle_collocate_pgf_product(
+          kernel(
               tl->orthorhombic, k->border_mask, func, ib->lmax[is], ib->lmin[is],
               jb->lmax[js], jb->lmin[js], ib->zet[is * ib->maxpgf + k->ipgf - 1],
               jb->zet[js * jb->maxpgf + k->jpgf - 1], ia == ja ? 1.0 : 2.0, y->dh,
@@ -1115,10 +1122,35 @@ void oracle_collocate_task_list(const oracle_task_list *tl, int func, int nlevel
   }
 }
 
+/* The oracle's integrate kernel behind the reference-style signature. */
+static void integrate_adapter(bool ortho, bool tau, int bm, int la_max, int la_min,
+                              int lb_max, int lb_min, double za, double zb,
+                              const double dh[3][3], const double dh_inv[3][3],
+                              const double ra[3], const double rab[3], const int ng[3],
+                              const int nl[3], const int sh[3], const int bw[3],
+                              double radius, int o1, int o2, int n1, int n2,
+                              const double *grid, double *hab, const double *pab,
+                              double *forces, double *virials, void *hdab, void *hadb,
+                              void *a_hdab) {
+  (void)hdab; (void)hadb; (void)a_hdab;
+  oracle_integrate_pgf_product(ortho, tau, bm, la_max, la_min, lb_max, lb_min, za, zb, dh,
+                               dh_inv, ra, rab, ng, nl, sh, bw, radius, o1, o2, n1, n2,
+                               grid, hab, pab, forces, virials);
+}
+
 void oracle_integrate_task_list(const oracle_task_list *tl, bool compute_tau,
                                 int natoms, int nlevels, const double *pab_blocks,
                                 const double *const *grids, double *hab_blocks,
                                 long hab_len, double *forces, double *virial) {
+  oracle_integrate_task_list_with(tl, compute_tau, natoms, nlevels, pab_blocks, grids,
+                                  hab_blocks, hab_len, forces, virial, integrate_adapter);
+}
+
+void oracle_integrate_task_list_with(const oracle_task_list *tl, bool compute_tau,
+                                     int natoms, int nlevels, const double *pab_blocks,
+                                     const double *const *grids, double *hab_blocks,
+                                     long hab_len, double *forces, double *virial,
+                                     oracle_integrate_fn kernel) {
   assert(nlevels == tl->nlevels && natoms == tl->natoms);
   assert(!(forces || virial) || pab_blocks);
   memset(hab_blocks, 0, sizeof(double) * hab_len);
@@ -1160,7 +1192,7 @@ void oracle_integrate_task_list(const oracle_task_list *tl, bool compute_tau,
             cur_is = is;
             cur_js = js;
           }
-          oracle_integrate_pgf_product(
+          kernel(
               tl->orthorhombic, compute_tau, k->border_mask, ib->lmax[is],
               ib->lmin[is], jb->lmax[js], jb->lmin[js],
               ib->zet[is * ib->maxpgf + k->ipgf - 1],
@@ -1169,7 +1201,7 @@ void oracle_integrate_task_list(const oracle_task_list *tl, bool compute_tau,
               y->shift_local, y->border_width, k->radius, (k->ipgf - 1) * nca,
               (k->jpgf - 1) * ncb, ncoa, ncob, grid, hab,
               (forces || virial) ? pab : NULL, forces ? fblk : NULL,
-              virial ? vblk : NULL);
+              virial ? vblk : NULL, NULL, NULL, NULL);
         }
         if (cur_is >= 0)
           contract_add(ib, jb, cur_is, cur_js, ia <= ja, hab, &hab_blocks[off]);

@@ -109,6 +109,41 @@ void oracle_integrate_task_list(const oracle_task_list *tl, bool compute_tau,
                                 const double *const *grids, double *hab_blocks,
                                 long hab_len, double *forces, double *virial);
 
+/* The same task-list drivers with a pluggable per-task kernel, so the CPU
+ * baseline can run the reference's own cpu-backend kernels
+ * (grid_cpu_collocate_pgf_product / grid_cpu_integrate_pgf_product, whose
+ * signatures these are, src/grid/cpu/grid_cpu_collocate.h and
+ * grid_cpu_integrate.h) under the oracle's OpenMP block loop. */
+typedef void (*oracle_collocate_fn)(
+    bool orthorhombic, int border_mask, int func, int la_max, int la_min,
+    int lb_max, int lb_min, double zeta, double zetb, double rscale,
+    const double dh[3][3], const double dh_inv[3][3], const double ra[3],
+    const double rab[3], const int npts_global[3], const int npts_local[3],
+    const int shift_local[3], const int border_width[3], double radius, int o1,
+    int o2, int n1, int n2, const double *pab, double *grid);
+
+typedef void (*oracle_integrate_fn)(
+    bool orthorhombic, bool compute_tau, int border_mask, int la_max,
+    int la_min, int lb_max, int lb_min, double zeta, double zetb,
+    const double dh[3][3], const double dh_inv[3][3], const double ra[3],
+    const double rab[3], const int npts_global[3], const int npts_local[3],
+    const int shift_local[3], const int border_width[3], double radius, int o1,
+    int o2, int n1, int n2, const double *grid, double *hab, const double *pab,
+    double *forces, double *virials, void *hdab, void *hadb, void *a_hdab);
+
+void oracle_collocate_task_list_with(const oracle_task_list *tl, int func,
+                                     int nlevels, const double *pab_blocks,
+                                     double *const *grids,
+                                     oracle_collocate_fn kernel);
+
+void oracle_integrate_task_list_with(const oracle_task_list *tl,
+                                     bool compute_tau, int natoms, int nlevels,
+                                     const double *pab_blocks,
+                                     const double *const *grids,
+                                     double *hab_blocks, long hab_len,
+                                     double *forces, double *virial,
+                                     oracle_integrate_fn kernel);
+
 /* Sum of oracle_task_points over every task of the list, for func's l-shift
  * (collocate) -- the task-point count used by the throughput metric. */
 long oracle_task_list_points(const oracle_task_list *tl, int lp_extra);

@@ -191,18 +191,24 @@ class TaskList:
             lib().oracle_free_task_list(C.c_void_p(self.h))
             self.h = None
 
-    def collocate(self, func, pab_blocks):
+    def collocate(self, func, pab_blocks, kernel=None):
+        """kernel: None (the oracle's own) or "ref" (the reference's cpu-backend
+        kernel grid_cpu_collocate_pgf_product from oracle/_ref)."""
         a = self.a
         grids = [np.zeros((int(n[2]), int(n[1]), int(n[0])))
                  for n in a.npts_local]
         ptrs = (_dp * a.nlevels)(*[_ptr(g) for g in grids])
-        lib().oracle_collocate_task_list(C.c_void_p(self.h), C.c_int(func),
-                                         C.c_int(a.nlevels), _ptr(_d(pab_blocks)),
-                                         ptrs)
+        args = [C.c_void_p(self.h), C.c_int(func), C.c_int(a.nlevels),
+                _ptr(_d(pab_blocks)), ptrs]
+        if kernel is None:
+            lib().oracle_collocate_task_list(*args)
+        else:
+            fn = C.cast(ref_lib(fast=True).grid_cpu_collocate_pgf_product, C.c_void_p)
+            lib().oracle_collocate_task_list_with(*args, fn)
         return grids
 
     def integrate(self, grids, pab_blocks=None, compute_tau=False,
-                  forces=False, virial=False, hab_len=None):
+                  forces=False, virial=False, hab_len=None, kernel=None):
         a = self.a
         hab_len = int(a.buffer_size if hab_len is None else hab_len)
         hab = np.zeros(hab_len)
@@ -211,10 +217,14 @@ class TaskList:
         gs = [_d(g) for g in grids]
         ptrs = (_dp * a.nlevels)(*[_ptr(g) for g in gs])
         pab = None if pab_blocks is None else _d(pab_blocks)
-        lib().oracle_integrate_task_list(
-            C.c_void_p(self.h), C.c_bool(compute_tau), C.c_int(a.natoms),
-            C.c_int(a.nlevels), _ptr(pab), ptrs, _ptr(hab), C.c_long(hab_len),
-            _ptr(f), _ptr(v))
+        args = [C.c_void_p(self.h), C.c_bool(compute_tau), C.c_int(a.natoms),
+                C.c_int(a.nlevels), _ptr(pab), ptrs, _ptr(hab), C.c_long(hab_len),
+                _ptr(f), _ptr(v)]
+        if kernel is None:
+            lib().oracle_integrate_task_list(*args)
+        else:
+            fn = C.cast(ref_lib(fast=True).grid_cpu_integrate_pgf_product, C.c_void_p)
+            lib().oracle_integrate_task_list_with(*args, fn)
         return hab, f, v
 
     def task_points(self):
