@@ -25,8 +25,8 @@
 #include "grid_internal.h"
 
 namespace gm {
-hipError_t launch_collocate(const PointParams &P, int nwaves, hipStream_t s);
-hipError_t launch_integrate(const PointParams &P, int max_lp, int nwaves, hipStream_t s);
+hipError_t launch_collocate(const PointParams &P, bool ortho, hipStream_t s);
+hipError_t launch_integrate(const PointParams &P, bool ortho, hipStream_t s);
 hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,
                              long long pab_len, const double *sphi, double *cart,
                              long long cart_len, int cach, int max_nsb, hipStream_t s);
@@ -315,6 +315,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
         y.dh_inv[i][j] = dh_inv[9 * l + 3 * i + j];
       }
     }
+    if ((long long)y.npts_local[0] * y.npts_local[1] * y.npts_local[2] >= (1LL << 31))
+      fatal("level %d: local grids of 2^31 points or more are not supported", l + 1);
   }
 
   // --- validate indices (the reference trusts its caller; a wrong index here
@@ -351,10 +353,24 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
       fatal("task %d: pgf out of range", t + 1);
   }
 
-  // --- order: level, block, iset, jset (src/grid/ref/grid_ref_task_list.c:26-37) ---
+  // --- order: kernel class (orthorhombic, general, too small to map), then
+  //     level, block, iset, jset (src/grid/ref/grid_ref_task_list.c:26-37) ---
+  std::vector<double> level_hmax(nlevels, 0.0);
+  for (int l = 0; l < nlevels; l++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        level_hmax[l] = fmax(level_hmax[l], fabs(tl->layouts[l].dh[i][j]));
+  std::vector<char> cls(ntasks);
+  for (int t = 0; t < ntasks; t++) {
+    if (2.0 * radius_list[t] < level_hmax[level_list[t] - 1])
+      cls[t] = 2;
+    else
+      cls[t] = (orthorhombic && border_mask_list[t] == 0) ? 0 : 1;
+  }
   std::vector<int> order(ntasks);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (cls[a] != cls[b]) return cls[a] < cls[b];
     if (level_list[a] != level_list[b]) return level_list[a] < level_list[b];
     if (block_num_list[a] != block_num_list[b]) return block_num_list[a] < block_num_list[b];
     if (iset_list[a] != iset_list[b]) return iset_list[a] < iset_list[b];
@@ -620,7 +636,8 @@ LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
     p.o_stage = p.o_cxrow + rowcap * p.np;
     const int lpb = lpb_bucket(lp);
     const int stage = (integrate && lpb <= 8) ? 64 * ((lpb + 1) * (lpb + 2) / 2 + lpb + 1) : 0;
-    const int phase_b = p.o_stage + stage - u;
+    p.o_row4 = (p.o_stage + stage + 1) & ~1;  // 16-byte aligned
+    const int phase_b = p.o_row4 + 2 * (rowcap + 1) - u;
     p.n_doubles = u + std::max(phase_a, phase_b);
     int i = 0;
     p.i_map = i; i += 3 * p.mapn;
@@ -632,20 +649,29 @@ LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
   };
   const int planes = std::max(1, tl->max_planes);
   const int limit = lds_limit();
-  layout(std::max(need_rows, 256), std::min(64, planes));
-  if (p.bytes > 48 * 1024) layout(std::max(need_rows, 128), std::min(16, planes));
-  if (p.bytes > 64 * 1024) layout(need_rows, std::min(4, planes));
+  // LDS decides how many workgroups share a CU (160 KiB each): keep the
+  // chunk arrays small; fewer planes per chunk only costs a few more syncs.
+  layout(std::max(need_rows, 96), std::min(12, planes));
+  if (p.bytes > 24 * 1024) layout(need_rows, std::min(4, planes));
   if (p.bytes > limit)
     fatal("task list needs %d bytes of LDS per task (lp=%d, cube half-width %d); "
           "the device limit is %d", p.bytes, lp, tl->max_cmax, limit);
-  (void)integrate;
   return p;
 }
 
-int nwaves_for(const TaskListImpl *) {
-  const char *e = getenv("GRID_MI355X_WAVES");
-  const int w = e ? atoi(e) : 2;
-  return (w == 1 || w == 2 || w == 4) ? w : 2;
+// Launches the orthorhombic and the general task ranges (tasks are sorted
+// by class: orthorhombic, general, skipped).
+void launch_ranges(const TaskListImpl *tl, PointParams P, hipStream_t s, bool integrate) {
+  const int ranges[2][2] = {{0, (int)tl->n_ortho}, {(int)tl->n_ortho, (int)tl->n_general}};
+  for (int c = 0; c < 2; c++) {
+    if (ranges[c][1] <= 0) continue;
+    P.task_base = ranges[c][0];
+    P.ntasks = ranges[c][1];
+    if (integrate)
+      HIP_CHECK(launch_integrate(P, c == 0, s));
+    else
+      HIP_CHECK(launch_collocate(P, c == 0, s));
+  }
 }
 
 void set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s) {
@@ -708,7 +734,7 @@ void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const 
   P.plan = make_plan(tl, ld, false);
   {
     TimedLaunch tm(0, s);
-    HIP_CHECK(launch_collocate(P, nwaves_for(tl), s));
+    launch_ranges(tl, P, s, false);
   }
   count_stats(tl, true, ld[0] + ld[2]);
 }
@@ -744,10 +770,14 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   P.do_forces = do_f;
   P.do_virial = do_v;
   P.plan = make_plan(tl, ld, true);
-  const int lpmax = P.plan.np - 1;
+  if (do_f && tl->n_skip > 0) {  // skipped tasks contribute no forces
+    const long long first = (long long)(tl->n_ortho + tl->n_general) * kTaskOutStride;
+    HIP_CHECK(launch_fill((double *)tl->d_task_out.p + first,
+                          (long long)tl->n_skip * kTaskOutStride, 0.0, s));
+  }
   {
     TimedLaunch tm(1, s);
-    HIP_CHECK(launch_integrate(P, lpmax, nwaves_for(tl), s));
+    launch_ranges(tl, P, s, true);
   }
   {
     TimedLaunch tm(4, s);
@@ -1237,7 +1267,11 @@ long long grid_mi355x_task_points(const grid_task_list *task_list) {
   P.count_only = 1;
   const int ld[4] = {0, 0, 0, 0};
   P.plan = make_plan(tl, ld, false);
-  HIP_CHECK(launch_collocate(P, nwaves_for(tl), s));
+  if (tl->n_skip > 0)
+    HIP_CHECK(launch_fill((double *)tl->d_task_out.p +
+                              (long long)(tl->n_ortho + tl->n_general) * kTaskOutStride,
+                          (long long)tl->n_skip * kTaskOutStride, 0.0, s));
+  launch_ranges(tl, P, s, false);
   std::vector<double> out((size_t)tl->ntasks * kTaskOutStride);
   HIP_CHECK(hipMemcpyAsync(out.data(), tl->d_task_out.p, sizeof(double) * out.size(),
                            hipMemcpyDeviceToHost, s));

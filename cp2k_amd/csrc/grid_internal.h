@@ -76,6 +76,7 @@ struct LdsPlan {
   int rowcap, plcap;
   int o_cab, o_alpha, o_cxyz, o_cijk, o_tab, o_cxy, o_cxrow, o_red;  // doubles
   int o_stage;   // per-column moment staging of the orthorhombic integrate
+  int o_row4;    // per-row int4 {first x, length, grid row base, plane<<16 | y+32768}
   int n_doubles;
   int i_map, i_row, i_plane, i_misc;  // ints, after the doubles
   int n_ints;
@@ -83,13 +84,12 @@ struct LdsPlan {
 };
 
 // Register-array bound of the point kernels for a maximum lp.
-inline int lpb_bucket(int lp) {
-  return lp <= 2 ? 2 : lp <= 4 ? 4 : lp <= 6 ? 6 : lp <= 8 ? 8 : lp <= 12 ? 12 : 16;
-}
+inline int lpb_bucket(int lp) { return lp <= 4 ? 4 : lp <= 8 ? 8 : lp <= 12 ? 12 : 16; }
 
 struct PointParams {
   const Task *tasks;
-  int ntasks;
+  int task_base;          // first task of the launched range
+  int ntasks;             // tasks in the launched range
   const Group *groups;
   const Level *levels;
   const double *cart;     // decontracted pab (collocate; integrate with forces)

@@ -392,10 +392,12 @@ struct Chunk {
   int *ri;    // row first i                            [rowcap+1]
   int *roff;  // row length, then exclusive offsets     [rowcap+1]
   int *misc;  // [0] planes, [1] rows, [2..] scan scratch
+  int4 *row4; // orthorhombic rows: {first i, length, grid row base, plane<<16 | j+32768}
 };
 
-__device__ __forceinline__ Chunk chunk_arrays(int *ib, const LdsPlan &pl) {
+__device__ __forceinline__ Chunk chunk_arrays(double *lds, int *ib, const LdsPlan &pl) {
   Chunk c;
+  c.row4 = reinterpret_cast<int4 *>(lds + pl.o_row4);
   int *p = ib + pl.i_plane;
   c.pk = p;
   c.prow = p + (pl.plcap + 1);
@@ -490,6 +492,13 @@ __device__ int build_chunk(const Task &T, const Layout &lay, bool ortho, int kbe
     C.rp[r] = p;
     C.ri[r] = i0;
     C.roff[r] = len;
+    if (ortho) {
+      // one 16-byte record per row for the column-owning point loops; the
+      // host guarantees level grids below 2^31 points
+      const int rowbase = (map[2 * pl.mapn + k + T.cmax] * lay.npts_local[1] +
+                           map[pl.mapn + j + T.cmax]) * lay.npts_local[0];
+      C.row4[r] = make_int4(i0, len, rowbase, (p << 16) | (j + 32768));
+    }
   }
   __syncthreads();
   // exclusive scan of row lengths -> offsets (segment per thread)
@@ -513,7 +522,7 @@ __device__ int build_chunk(const Task &T, const Layout &lay, bool ortho, int kbe
 __device__ __forceinline__ void stage_task(const PointParams &P, Task &sT, Level &sL, int tid,
                                            int nthr) {
   static_assert(sizeof(Task) % 4 == 0 && sizeof(Level) % 4 == 0, "word copies");
-  const int *src = reinterpret_cast<const int *>(&P.tasks[blockIdx.x]);
+  const int *src = reinterpret_cast<const int *>(&P.tasks[P.task_base + blockIdx.x]);
   int *dst = reinterpret_cast<int *>(&sT);
   for (int e = tid; e < (int)(sizeof(Task) / 4); e += nthr) dst[e] = src[e];
   __syncthreads();
@@ -546,7 +555,9 @@ __device__ __forceinline__ XSlots x_slots(const Task &T, int tid) {
   return X;
 }
 
-template <int NW, int LPB>
+// ORTHO: the launch covers only orthorhombic (true) or only general (false)
+// tasks, so each instantiation carries one code path.
+template <int NW, int LPB, bool ORTHO>
 __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
   extern __shared__ double lds[];
   __shared__ Task sT;
@@ -555,13 +566,14 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
   stage_task(P, sT, sL, tid, nthr);
   const Task &T = sT;
   if (T.flags & kTaskSkip) {
-    if (P.count_only && tid == 0) P.task_out[(long long)blockIdx.x * kTaskOutStride] = 0.0;
+    if (P.count_only && tid == 0)
+      P.task_out[(long long)(P.task_base + blockIdx.x) * kTaskOutStride] = 0.0;
     return;
   }
   const LdsPlan &pl = P.plan;
   const Level &L = sL;
   const Layout &lay = L.lay;
-  const bool ortho = (T.flags & kTaskOrtho) != 0;
+  constexpr bool ortho = ORTHO;
   double *cab = lds + pl.o_cab;
   double *alpha = lds + pl.o_alpha;
   double *cxyz = lds + pl.o_cxyz;
@@ -571,7 +583,7 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
   double *cxrow = lds + pl.o_cxrow;
   int *ib = reinterpret_cast<int *>(lds + pl.n_doubles);
   int *map = ib + pl.i_map;
-  const Chunk C = chunk_arrays(ib, pl);
+  const Chunk C = chunk_arrays(lds, ib, pl);
 
   const int la = T.la_max + P.ld[0], lb = T.lb_max + P.ld[2];
   const int la_min = max(T.la_min + P.ld[1], 0), lb_min = max(T.lb_min + P.ld[3], 0);
@@ -707,15 +719,14 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
           px[l] = (iv && l <= lp) ? tab[(0 * NP + l) * EXT + i + cmax] : 0.0;
         const long long ig = iv ? mapx[i + cmax] : 0;
         for (int r = X.slot; r < nrows; r += X.nslots) {
-          const int i0 = C.ri[r];
-          if (!iv || i < i0 || i >= i0 + (C.roff[r + 1] - C.roff[r])) continue;
+          const int4 m = C.row4[r];
+          if (!iv || i < m.x || i >= m.x + m.y) continue;
           const double *cr = &cxrow[r * NP];
           double v = 0.0;
 #pragma unroll
           for (int l = 0; l <= LPB; l++)
             if (l <= lp) v = fma(cr[l], px[l], v);
-          const int j = C.rj[r], k = C.pk[C.rp[r]];
-          const long long gidx = ((long long)mapz[k + cmax] * ny + mapy[j + cmax]) * nx + ig;
+          const long long gidx = (long long)m.z + ig;
           if (gidx < L.npts) atomicAdd(&L.grid[gidx], v);
         }
       }
@@ -760,7 +771,7 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
     kbeg += npl;
   }
   if (P.count_only && tid == 0)
-    P.task_out[(long long)blockIdx.x * kTaskOutStride] = counted;
+    P.task_out[(long long)(P.task_base + blockIdx.x) * kTaskOutStride] = counted;
 }
 
 // ---------------------------------------------------------------------------
@@ -853,7 +864,8 @@ __device__ double get_vb(Ang a, Ang b, int i, int j, double za, double zb,
 }
 #undef TAU_SUM
 
-template <int NW, int LPB>
+// FULL = false: hab only, no tau (forces/virial/tau code compiled out).
+template <int NW, int LPB, bool FULL, bool ORTHO>
 __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   extern __shared__ double lds[];
   __shared__ Task sT;
@@ -861,7 +873,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   const int tid = threadIdx.x, nthr = 64 * NW;
   stage_task(P, sT, sL, tid, nthr);
   const Task &T = sT;
-  double *out = P.task_out + (long long)blockIdx.x * kTaskOutStride;
+  double *out = P.task_out + (long long)(P.task_base + blockIdx.x) * kTaskOutStride;
   if (T.flags & kTaskSkip) {
     if (P.do_forces && tid < kTaskOutStride) out[tid] = 0.0;
     return;
@@ -869,7 +881,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   const LdsPlan &pl = P.plan;
   const Level &L = sL;
   const Layout &lay = L.lay;
-  const bool ortho = (T.flags & kTaskOrtho) != 0;
+  constexpr bool ortho = ORTHO;
   double *cab = lds + pl.o_cab;
   double *alpha = lds + pl.o_alpha;
   double *cxyz = lds + pl.o_cxyz;
@@ -880,7 +892,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   double *red = lds + pl.o_red;
   int *ib = reinterpret_cast<int *>(lds + pl.n_doubles);
   int *map = ib + pl.i_map;
-  const Chunk C = chunk_arrays(ib, pl);
+  const Chunk C = chunk_arrays(lds, ib, pl);
 
   // l ranges of cab: process_get_ldiffs, grid_process_vab.h:222-251
   const int la = T.la_max + P.ld[0], lb = T.lb_max + P.ld[2];
@@ -944,15 +956,15 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
           for (int l = 0; l <= LPB; l++) R[l] = 0.0;
         };
         for (int r = X.slot; r < nrows; r += X.nslots) {
-          const int p = C.rp[r];
+          const int4 m = C.row4[r];
+          const int p = m.w >> 16;
           if (p != cur) {
             if (cur >= 0) flush(cur);
             cur = p;
           }
-          const int i0 = C.ri[r];
-          if (!iv || i < i0 || i >= i0 + (C.roff[r + 1] - C.roff[r])) continue;
-          const int j = C.rj[r], k = C.pk[p];
-          const long long gidx = ((long long)mapz[k + cmax] * ny + mapy[j + cmax]) * nx + ig;
+          if (!iv || i < m.x || i >= m.x + m.y) continue;
+          const int j = (m.w & 0xffff) - 32768;
+          const long long gidx = (long long)m.z + ig;
           const double g = (gidx < L.npts) ? grid[gidx] : 0.0;
 #pragma unroll
           for (int ly = 0; ly <= LPB; ly++)
@@ -1092,7 +1104,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   // ---- hab, forces, virial (grid_ref_integrate.c:132-171) ----
   const Group &G = P.groups[T.group];
   const CabGet cg{cab, n1};
-  const bool tau = P.compute_tau != 0;
+  const bool tau = FULL && P.compute_tau != 0;
   const double za = T.zeta, zb = T.zetb;
   const int a0 = ncoset(T.la_min - 1), na = ncoset(T.la_max) - a0;
   const int b0 = ncoset(T.lb_min - 1), nb = ncoset(T.lb_max) - b0;
@@ -1103,7 +1115,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
     const Ang A = orb(ia), B = orb(jb);
     const long long q = G.cart_off + (long long)(T.o2 + jb) * G.ncoa + T.o1 + ia;
     if (q < P.cart_len) atomicAdd(&P.hab_cart[q], get_hab(A, B, za, zb, cg, tau));
-    if (P.do_forces) {
+    if (FULL && P.do_forces) {
       const double pv = (q < P.cart_len) ? P.cart[q] : 0.0;
       for (int i = 0; i < 3; i++) {
         f[i] += pv * get_fa(A, B, i, za, zb, cg, tau);
@@ -1118,7 +1130,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
       }
     }
   }
-  if (P.do_forces) {
+  if (FULL && P.do_forces) {
     const int lane = tid & 63, w = tid >> 6;
     for (int c = 0; c < kTaskOutStride; c++) {
       const double x = wave_sum(c < 6 ? f[c] : v[c - 6]);
@@ -1285,51 +1297,48 @@ static hipError_t launch_task_kernel(K kernel, const PointParams &P, int nthread
   return hipGetLastError();
 }
 
-template <int NW>
-static hipError_t launch_collocate_nw(const PointParams &P, int lp, size_t lds, hipStream_t s) {
-  const int n = 64 * NW;
+template <bool ORTHO>
+static hipError_t launch_collocate_kind(const PointParams &P, int lp, size_t lds, hipStream_t s) {
   switch (lpb_bucket(lp)) {
-    case 2: return launch_task_kernel(collocate_kernel<NW, 2>, P, n, lds, s);
-    case 4: return launch_task_kernel(collocate_kernel<NW, 4>, P, n, lds, s);
-    case 6: return launch_task_kernel(collocate_kernel<NW, 6>, P, n, lds, s);
-    case 8: return launch_task_kernel(collocate_kernel<NW, 8>, P, n, lds, s);
-    case 12: return launch_task_kernel(collocate_kernel<NW, 12>, P, n, lds, s);
-    default: return launch_task_kernel(collocate_kernel<NW, 16>, P, n, lds, s);
+    case 4: return launch_task_kernel(collocate_kernel<2, 4, ORTHO>, P, 128, lds, s);
+    case 8: return launch_task_kernel(collocate_kernel<2, 8, ORTHO>, P, 128, lds, s);
+    case 12: return launch_task_kernel(collocate_kernel<2, 12, ORTHO>, P, 128, lds, s);
+    default: return launch_task_kernel(collocate_kernel<2, 16, ORTHO>, P, 128, lds, s);
   }
 }
 
-hipError_t launch_collocate(const PointParams &P, int nwaves, hipStream_t s) {
+// P.task_base / P.ntasks select the range of the task array; ortho says
+// which kind of tasks the range holds.
+hipError_t launch_collocate(const PointParams &P, bool ortho, hipStream_t s) {
   const size_t lds = P.plan.bytes;
   const int lp = P.plan.np - 1;
   if (lp > 16) return hipErrorInvalidValue;
-  switch (nwaves) {
-    case 1: return launch_collocate_nw<1>(P, lp, lds, s);
-    case 2: return launch_collocate_nw<2>(P, lp, lds, s);
-    default: return launch_collocate_nw<4>(P, lp, lds, s);
-  }
+  if (P.ntasks <= 0) return hipSuccess;
+  return ortho ? launch_collocate_kind<true>(P, lp, lds, s)
+               : launch_collocate_kind<false>(P, lp, lds, s);
 }
 
-template <int NW>
-static hipError_t launch_integrate_nw(const PointParams &P, int lp, size_t lds, hipStream_t s) {
-  const int n = 64 * NW;
+template <bool FULL, bool ORTHO>
+static hipError_t launch_integrate_kind(const PointParams &P, int lp, size_t lds, hipStream_t s) {
   switch (lpb_bucket(lp)) {
-    case 2: return launch_task_kernel(integrate_kernel<NW, 2>, P, n, lds, s);
-    case 4: return launch_task_kernel(integrate_kernel<NW, 4>, P, n, lds, s);
-    case 6: return launch_task_kernel(integrate_kernel<NW, 6>, P, n, lds, s);
-    case 8: return launch_task_kernel(integrate_kernel<NW, 8>, P, n, lds, s);
-    case 12: return launch_task_kernel(integrate_kernel<NW, 12>, P, n, lds, s);
-    default: return launch_task_kernel(integrate_kernel<NW, 16>, P, n, lds, s);
+    case 4: return launch_task_kernel(integrate_kernel<2, 4, FULL, ORTHO>, P, 128, lds, s);
+    case 8: return launch_task_kernel(integrate_kernel<2, 8, FULL, ORTHO>, P, 128, lds, s);
+    case 12: return launch_task_kernel(integrate_kernel<2, 12, FULL, ORTHO>, P, 128, lds, s);
+    default: return launch_task_kernel(integrate_kernel<2, 16, FULL, ORTHO>, P, 128, lds, s);
   }
 }
 
-hipError_t launch_integrate(const PointParams &P, int max_lp, int nwaves, hipStream_t s) {
+hipError_t launch_integrate(const PointParams &P, bool ortho, hipStream_t s) {
   const size_t lds = P.plan.bytes;
-  if (max_lp > 16) return hipErrorInvalidValue;
-  switch (nwaves) {
-    case 1: return launch_integrate_nw<1>(P, max_lp, lds, s);
-    case 2: return launch_integrate_nw<2>(P, max_lp, lds, s);
-    default: return launch_integrate_nw<4>(P, max_lp, lds, s);
-  }
+  const int lp = P.plan.np - 1;
+  if (lp > 16) return hipErrorInvalidValue;
+  if (P.ntasks <= 0) return hipSuccess;
+  const bool full = P.do_forces || P.compute_tau;
+  if (ortho)
+    return full ? launch_integrate_kind<true, true>(P, lp, lds, s)
+                : launch_integrate_kind<false, true>(P, lp, lds, s);
+  return full ? launch_integrate_kind<true, false>(P, lp, lds, s)
+              : launch_integrate_kind<false, false>(P, lp, lds, s);
 }
 
 hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,
