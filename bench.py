@@ -40,6 +40,10 @@ def parse():
     p.add_argument("--forces", action="store_true", help="integrate also forces+virial")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="before the steps, run 3 device-to-device copies of MIB MiB: "
+                        "a plain copy kernel of known size that scales the HBM "
+                        "byte counters of a rocprofv3 --pmc run")
     return p.parse_args()
 
 
@@ -138,6 +142,14 @@ def main():
         gt.collocate_dev(100, pab_d, rho_d)
         gt.integrate_dev(V_d, hab_d, pab=pab_d if opt.forces else None, forces=f_d,
                          virial=v_d)
+
+    if opt.copy_calibration:
+        n = opt.copy_calibration * (1 << 20) // 8
+        src, dst = grid.DeviceArray((n,)), grid.DeviceArray((n,))
+        for _ in range(3):
+            dst.copy_from(src)
+        src.free()
+        dst.free()
 
     for _ in range(opt.warmup):
         step()

@@ -38,6 +38,8 @@ hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *ta
                                 double *forces, int natoms, hipStream_t s);
 hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
                                 hipStream_t s);
+hipError_t launch_collocate_wave(const PointParams &P, int lpb, int blocks, hipStream_t s);
+hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 }  // namespace gm
 
 using namespace gm;
@@ -88,6 +90,7 @@ struct LibState {
   hipStream_t user_stream = nullptr;
   bool stream_set = false;
   int precision = -1;
+  int wave_kernels = -1;  // wave-per-task kernels for orthorhombic tasks
   bool timing = false;
   double kernel_ms[kNumKernelIds] = {};
   long kernel_n[kNumKernelIds] = {};
@@ -108,6 +111,16 @@ int precision() {
     S.precision = (e && atoi(e) == 1) ? GRID_MI355X_FP32_MIXED : GRID_MI355X_FP64;
   }
   return S.precision;
+}
+
+// GRID_MI355X_WAVE=0 routes every task to the per-task kernels (comparison
+// runs); the default uses the wave-per-task kernels wherever they apply.
+bool wave_kernels() {
+  if (S.wave_kernels < 0) {
+    const char *e = getenv("GRID_MI355X_WAVE");
+    S.wave_kernels = (e && atoi(e) == 0) ? 0 : 1;
+  }
+  return S.wave_kernels == 1;
 }
 
 hipEvent_t get_event() {
@@ -223,6 +236,15 @@ void integrate_ldiffs(bool forces, bool virial, bool tau, int d[4]) {
 // Task list
 // ---------------------------------------------------------------------------
 namespace {
+// Orthorhombic tasks of one level: [begin, end) in the sorted task array,
+// sorted by lp0 = la_max + lb_max; lp_start[v] = first task with lp0 >= v.
+constexpr int kLp0Slots = 12;
+struct OrthoLevel {
+  int begin = 0, end = 0;
+  int max_ext = 0;  // largest cube extent 2 - 2 lo over the three axes
+  int lp_start[kLp0Slots] = {};
+};
+
 struct TaskListImpl {
   static constexpr unsigned kMagic = 0x6d693335u;
   unsigned magic = kMagic;
@@ -243,6 +265,8 @@ struct TaskListImpl {
   long long block_end = 0;  // max over groups of block offset + block size
   long n_ortho = 0, n_general = 0, n_skip = 0;
   long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
+  std::vector<OrthoLevel> olev;   // per level
+  std::vector<signed char> lp0s;  // lp0 per sorted task
   long long points = -1;
   // per-call scratch when the caller's buffers have no device mirror
   DevBuf pab_dev, hab_dev, forces_dev;
@@ -354,14 +378,17 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   }
 
   // --- order: kernel class (orthorhombic, general, too small to map), then
-  //     level, block, iset, jset (src/grid/ref/grid_ref_task_list.c:26-37) ---
+  //     level, la_max + lb_max (one polynomial degree bound per launch), block,
+  //     iset, jset (src/grid/ref/grid_ref_task_list.c:26-37) ---
   std::vector<double> level_hmax(nlevels, 0.0);
   for (int l = 0; l < nlevels; l++)
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++)
         level_hmax[l] = fmax(level_hmax[l], fabs(tl->layouts[l].dh[i][j]));
-  std::vector<char> cls(ntasks);
+  std::vector<char> cls(ntasks), lpk(ntasks);
   for (int t = 0; t < ntasks; t++) {
+    lpk[t] = (char)(basis_sets[atom_kinds[iatom_list[t] - 1] - 1]->lmax[iset_list[t] - 1] +
+                    basis_sets[atom_kinds[jatom_list[t] - 1] - 1]->lmax[jset_list[t] - 1]);
     if (2.0 * radius_list[t] < level_hmax[level_list[t] - 1])
       cls[t] = 2;
     else
@@ -372,6 +399,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     if (cls[a] != cls[b]) return cls[a] < cls[b];
     if (level_list[a] != level_list[b]) return level_list[a] < level_list[b];
+    if (lpk[a] != lpk[b]) return lpk[a] < lpk[b];
     if (block_num_list[a] != block_num_list[b]) return block_num_list[a] < block_num_list[b];
     if (iset_list[a] != iset_list[b]) return iset_list[a] < iset_list[b];
     return jset_list[a] < jset_list[b];
@@ -565,6 +593,25 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   }
   (void)max_set_l;
 
+  // --- per-level ranges of the orthorhombic tasks ---
+  tl->olev.assign(nlevels, OrthoLevel{});
+  tl->lp0s.resize(ntasks);
+  for (int s = 0; s < ntasks; s++) tl->lp0s[s] = (signed char)(tasks[s].la_max + tasks[s].lb_max);
+  for (int s = 0; s < ntasks; s++) {
+    const Task &T = tasks[s];
+    if (!(T.flags & kTaskOrtho)) continue;
+    OrthoLevel &o = tl->olev[T.level];
+    if (o.end == 0) o.begin = s;
+    o.end = s + 1;
+    for (int d = 0; d < 3; d++) o.max_ext = std::max(o.max_ext, 2 - 2 * T.lo[d]);
+  }
+  for (auto &o : tl->olev)
+    for (int v = 0; v < kLp0Slots; v++) {
+      int s = o.begin;
+      while (s < o.end && tl->lp0s[s] < v) s++;
+      o.lp_start[v] = s;
+    }
+
   // --- force CSR: atom -> (2*task + side) ---
   std::vector<int> fptr(natoms + 1, 0), fent(2 * (size_t)ntasks);
   for (int s = 0; s < ntasks; s++) {
@@ -659,18 +706,115 @@ LdsPlan make_plan(const TaskListImpl *tl, const int ld[4], bool integrate) {
   return p;
 }
 
+int num_cus() {
+  static int n = 0;
+  if (!n) {
+    int dev = 0, v = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+    n = v > 0 ? v : 1;
+  }
+  return n;
+}
+
+enum LaunchMode { kCollocate = 0, kIntegrate = 1, kCount = 2 };
+
+// LDS plan of a wave-per-task launch over orthorhombic tasks of one level
+// with lp <= `lp`.  Collocation keeps the whole level grid in LDS when it
+// fits next to at least four wave regions.
+WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[4], int mode) {
+  const OrthoLevel &o = tl->olev[level];
+  WavePlan w{};
+  w.level = level;
+  w.np = lp + 1;
+  w.ext = o.max_ext;
+  w.i0n = o.max_ext / 2;
+  // cab [ncoset(la)][ncoset(lb)] + alpha [3][lb+1][la+1][lp+1] for la + lb <= lp
+  int n_setup = 0;
+  for (int la = 0; la <= tl->max_la + ld[0]; la++)
+    for (int lb = 0; lb <= tl->max_lb + ld[2] && la + lb <= lp; lb++)
+      n_setup = std::max(n_setup, ncoset(la) * ncoset(lb) + 3 * (la + 1) * (lb + 1) * w.np);
+  w.o_cxyz = 0;
+  w.o_u = w.np * w.np * w.np;
+  const int nd = w.o_u + std::max(n_setup, 3 * w.ext * w.np);
+  w.o_i0 = 8 * nd;
+  w.o_col = w.o_i0 + ((w.i0n * w.i0n + 3) & ~3);
+  w.wave_bytes = (w.o_col + 4 * (w.ext + 2) + 15) & ~15;
+  const int limit = lds_limit();
+  if (mode == kCollocate) {
+    const long long npts = (long long)tl->npts_local[3 * level] * tl->npts_local[3 * level + 1] *
+                           tl->npts_local[3 * level + 2];
+    for (int nw : {8, 6, 4})
+      if (8 * npts + (long long)nw * w.wave_bytes <= limit) {
+        w.window = (int)npts;
+        w.nwaves = nw;
+        break;
+      }
+  }
+  if (!w.window) {
+    w.nwaves = 1;
+    for (int nw : {4, 2})
+      if (nw * w.wave_bytes <= limit) {
+        w.nwaves = nw;
+        break;
+      }
+  }
+  w.bytes = 8 * w.window + w.nwaves * w.wave_bytes;
+  if (w.bytes > limit)
+    fatal("level %d: %d bytes of LDS per workgroup needed, the device has %d", level + 1,
+          w.bytes, limit);
+  return w;
+}
+
+// Orthorhombic tasks, level by level: wave-per-task kernels for lp <= 6 and
+// cube extents <= kWaveMaxExt, the per-task kernels for the rest.
+void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+  const int sh = P.ld[0] + P.ld[2];
+  for (int l = 0; l < tl->nlevels; l++) {
+    const OrthoLevel &o = tl->olev[l];
+    if (o.end <= o.begin) continue;
+    auto first = [&](int v) { return o.lp_start[std::min(std::max(v, 0), kLp0Slots - 1)]; };
+    int c1 = first(5 - sh), c2 = first(7 - sh);  // lp <= 4 | lp 5..6 | larger
+    if (mode == kCount) c1 = c2 = o.end;          // counting ignores lp
+    if (!wave_kernels() || o.max_ext > kWaveMaxExt) c1 = c2 = o.begin;
+    const int cuts[3][2] = {{o.begin, c1}, {c1, c2}, {c2, o.end}};
+    for (int r = 0; r < 2; r++) {
+      const int a = cuts[r][0], b = cuts[r][1];
+      if (b <= a) continue;
+      const int lpb = r == 0 ? 4 : 6;
+      const int lp = mode == kCount ? 0 : std::min(lpb, tl->lp0s[b - 1] + sh);
+      PointParams Q = P;
+      Q.task_base = a;
+      Q.ntasks = b - a;
+      Q.wp = make_wave_plan(tl, l, lp, P.ld, mode);
+      if (mode == kIntegrate) {
+        HIP_CHECK(launch_integrate_wave(Q, lpb, s));
+      } else {
+        int blocks = (Q.ntasks + Q.wp.nwaves - 1) / Q.wp.nwaves;
+        if (Q.wp.window) {
+          const int per_cu = std::max(1, lds_limit() / Q.wp.bytes);
+          blocks = std::min(blocks, num_cus() * per_cu);
+        }
+        HIP_CHECK(launch_collocate_wave(Q, lpb, blocks, s));
+      }
+    }
+    if (o.end > cuts[2][0]) {
+      PointParams Q = P;
+      Q.task_base = cuts[2][0];
+      Q.ntasks = o.end - cuts[2][0];
+      HIP_CHECK(mode == kIntegrate ? launch_integrate(Q, true, s) : launch_collocate(Q, true, s));
+    }
+  }
+}
+
 // Launches the orthorhombic and the general task ranges (tasks are sorted
 // by class: orthorhombic, general, skipped).
-void launch_ranges(const TaskListImpl *tl, PointParams P, hipStream_t s, bool integrate) {
-  const int ranges[2][2] = {{0, (int)tl->n_ortho}, {(int)tl->n_ortho, (int)tl->n_general}};
-  for (int c = 0; c < 2; c++) {
-    if (ranges[c][1] <= 0) continue;
-    P.task_base = ranges[c][0];
-    P.ntasks = ranges[c][1];
-    if (integrate)
-      HIP_CHECK(launch_integrate(P, c == 0, s));
-    else
-      HIP_CHECK(launch_collocate(P, c == 0, s));
+void launch_ranges(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+  if (tl->n_ortho > 0) launch_ortho(tl, P, s, mode);
+  if (tl->n_general > 0) {
+    P.task_base = (int)tl->n_ortho;
+    P.ntasks = (int)tl->n_general;
+    HIP_CHECK(mode == kIntegrate ? launch_integrate(P, false, s) : launch_collocate(P, false, s));
   }
 }
 
@@ -734,7 +878,7 @@ void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const 
   P.plan = make_plan(tl, ld, false);
   {
     TimedLaunch tm(0, s);
-    launch_ranges(tl, P, s, false);
+    launch_ranges(tl, P, s, kCollocate);
   }
   count_stats(tl, true, ld[0] + ld[2]);
 }
@@ -777,7 +921,7 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   }
   {
     TimedLaunch tm(1, s);
-    launch_ranges(tl, P, s, true);
+    launch_ranges(tl, P, s, kIntegrate);
   }
   {
     TimedLaunch tm(4, s);
@@ -1271,7 +1415,7 @@ long long grid_mi355x_task_points(const grid_task_list *task_list) {
     HIP_CHECK(launch_fill((double *)tl->d_task_out.p +
                               (long long)(tl->n_ortho + tl->n_general) * kTaskOutStride,
                           (long long)tl->n_skip * kTaskOutStride, 0.0, s));
-  launch_ranges(tl, P, s, false);
+  launch_ranges(tl, P, s, kCount);
   std::vector<double> out((size_t)tl->ntasks * kTaskOutStride);
   HIP_CHECK(hipMemcpyAsync(out.data(), tl->d_task_out.p, sizeof(double) * out.size(),
                            hipMemcpyDeviceToHost, s));

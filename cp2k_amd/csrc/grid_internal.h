@@ -86,6 +86,24 @@ struct LdsPlan {
 // Register-array bound of the point kernels for a maximum lp.
 inline int lpb_bucket(int lp) { return lp <= 4 ? 4 : lp <= 8 ? 8 : lp <= 12 ? 12 : 16; }
 
+// Wave-per-task kernels (orthorhombic tasks of one grid level): each wave of
+// a workgroup owns an LDS region of wave_bytes; collocation into a level
+// whose whole grid fits into LDS keeps that grid ("window", in doubles) at
+// the start of LDS and adds it to HBM once per workgroup.
+constexpr int kWaveMaxExt = 64;  // cube extent per axis handled by one wave
+struct WavePlan {
+  int level;       // grid level of the launch
+  int np;          // polynomial stride (max lp + 1)
+  int ext;         // per-axis table stride (max cube extent 2 - 2 lo)
+  int i0n;         // stride of the row-start table (max 1 - lo)
+  int nwaves;      // waves per workgroup
+  int wave_bytes;  // LDS bytes per wave
+  int window;      // doubles of the LDS level window (0: accumulate in HBM)
+  int o_cxyz, o_u; // doubles: cxyz [np^3]; union of {cab, alpha} and tables [3][ext][np]
+  int o_i0, o_col; // bytes: int8 row starts [i0n][i0n]; int column starts [ext + 1]
+  int bytes;       // dynamic LDS per workgroup
+};
+
 struct PointParams {
   const Task *tasks;
   int task_base;          // first task of the launched range
@@ -101,6 +119,7 @@ struct PointParams {
   int compute_tau, do_forces, do_virial;
   int count_only;         // count task points into task_out[t*15]
   LdsPlan plan;
+  WavePlan wp;
 };
 
 }  // namespace gm

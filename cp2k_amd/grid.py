@@ -171,6 +171,13 @@ class DeviceArray:
                                  C.c_size_t(self.nbytes), C.c_int(2))
         return a
 
+    def copy_from(self, other):
+        """Device-to-device copy (HIP's blit kernel) of another DeviceArray."""
+        if other.nbytes != self.nbytes:
+            raise ValueError(f"copy of {other.nbytes} bytes into {self.nbytes}")
+        lib().grid_mi355x_memcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr),
+                                 C.c_size_t(self.nbytes), C.c_int(3))
+
     def data_ptr(self):
         return self.ptr
 
