@@ -15,6 +15,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -38,8 +40,10 @@ hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *ta
                                 double *forces, int natoms, hipStream_t s);
 hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
                                 hipStream_t s);
+hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s);
 hipError_t launch_collocate_wave(const PointParams &P, int lpb, int blocks, hipStream_t s);
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
+hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
 }  // namespace gm
 
 using namespace gm;
@@ -241,9 +245,17 @@ namespace {
 constexpr int kLp0Slots = 12;
 struct OrthoLevel {
   int begin = 0, end = 0;
-  int max_ext = 0;  // largest cube extent 2 - 2 lo over the three axes
+  int max_ext = 0;      // largest cube extent 2 - 2 lo over the three axes
+  bool classed = true;  // every task has a geometry class
+  long long points = 0;  // task points of the level (classed levels)
   int lp_start[kLp0Slots] = {};
 };
+
+// Orthorhombic row start, src/grid/ref/grid_ref_collint.h:147 / :49 (the
+// device copy is ortho_start in grid_device.h).
+inline int ortho_start_host(double remain, double hinv) {
+  return (int)ceil(-1e-8 - sqrt(fmax(0.0, remain)) * hinv);
+}
 
 struct TaskListImpl {
   static constexpr unsigned kMagic = 0x6d693335u;
@@ -255,6 +267,7 @@ struct TaskListImpl {
   std::vector<int> npts_local;  // [nlevels][3]
   // device-resident description
   DevBuf d_tasks, d_groups, d_sphi, d_fptr, d_fent, d_levels, d_cart, d_hab_cart, d_task_out;
+  DevBuf d_geo, d_geo_cols, d_geo_kmin, d_cxyz;  // geometry classes, per-task cxyz
   Level *h_levels = nullptr;  // pinned staging for the per-call level table
   long long cart_len = 0;
   // maxima for the LDS plan
@@ -267,6 +280,8 @@ struct TaskListImpl {
   long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
   std::vector<OrthoLevel> olev;   // per level
   std::vector<signed char> lp0s;  // lp0 per sorted task
+  int ngeo = 0;
+  long long class_points = 0;     // task points of the tasks with a geometry class
   long long points = -1;
   // per-call scratch when the caller's buffers have no device mirror
   DevBuf pab_dev, hab_dev, forces_dev;
@@ -274,7 +289,8 @@ struct TaskListImpl {
 
   ~TaskListImpl() {
     DevBuf *bufs[] = {&d_tasks, &d_groups, &d_sphi, &d_fptr, &d_fent, &d_levels,
-                      &d_cart, &d_hab_cart, &d_task_out, &pab_dev, &hab_dev, &forces_dev};
+                      &d_cart, &d_hab_cart, &d_task_out, &pab_dev, &hab_dev, &forces_dev,
+                      &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
     if (h_levels) (void)hipHostFree(h_levels);
@@ -593,6 +609,77 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   }
   (void)max_set_l;
 
+  // --- geometry classes: orthorhombic tasks of one level with the same
+  //     discretised radius share their truncation region (it does not depend
+  //     on the centre).  Per class: the columns (i, j) of the central plane,
+  //     j-major, with the first plane kl of each column (planes kl .. 1 - kl),
+  //     from the bounds of src/grid/ref/grid_ref_collint.h:139-167 ---
+  std::vector<GeoClass> geo;
+  std::vector<int> geo_cols, geo_kmin;
+  std::map<std::pair<int, unsigned long long>, int> class_of;
+  tl->class_points = 0;
+  const int kNoRow = std::numeric_limits<int>::max();
+  for (int s = 0; s < ntasks; s++) {
+    Task &T = tasks[s];
+    T.geo = -1;
+    if (!(T.flags & kTaskOrtho)) continue;
+    if (2 - 2 * std::min({T.lo[0], T.lo[1], T.lo[2]}) > kWaveMaxExt) continue;
+    unsigned long long bits;
+    memcpy(&bits, &T.disr, sizeof bits);
+    const auto key = std::make_pair(T.level, bits);
+    auto it = class_of.find(key);
+    if (it == class_of.end()) {
+      const Layout &y = tl->layouts[T.level];
+      GeoClass G{};
+      G.col_off = (int)geo_cols.size();
+      G.grp_off = (int)geo_kmin.size();
+      for (int d = 0; d < 3; d++) G.lo[d] = T.lo[d];
+      const int nkz = 1 - T.lo[2], njy = 1 - T.lo[1];
+      std::vector<int> i0((size_t)nkz * njy);
+      for (int kz = 0; kz < nkz; kz++) {
+        const double kr = (T.lo[2] + kz) * y.dh[2][2];
+        const double krem = T.disr * T.disr - kr * kr;
+        const int js = ortho_start_host(krem, y.dh_inv[1][1]);
+        for (int jy = 0; jy < njy; jy++) {
+          const int j1 = T.lo[1] + jy;
+          int v = kNoRow;
+          if (j1 >= js) {
+            const double jr = j1 * y.dh[1][1];
+            const double jrem = krem - jr * jr;
+            v = ortho_start_host(jrem, y.dh_inv[0][0]);
+          }
+          i0[(size_t)kz * njy + jy] = v;
+        }
+      }
+      for (int j = T.lo[1]; j <= 1 - T.lo[1]; j++) {
+        const int jy = (j <= 0 ? j : 1 - j) - T.lo[1];
+        const int ic = i0[(size_t)(nkz - 1) * njy + jy];
+        if (ic == kNoRow) continue;
+        for (int i = ic; i <= 1 - ic; i++) {
+          const int i1 = i <= 0 ? i : 1 - i;
+          int kz = 0;
+          while (i0[(size_t)kz * njy + jy] > i1) kz++;  // the central plane holds it
+          for (int z = kz; z < nkz; z++)
+            if (i0[(size_t)z * njy + jy] > i1)
+              fatal("truncation region of level %d is not convex in z", T.level + 1);
+          geo_cols.push_back((i - T.lo[0]) | ((j - T.lo[1]) << 8) | (kz << 16));
+          G.npts += 2 - 2 * (T.lo[2] + kz);
+        }
+      }
+      G.ncol = (int)geo_cols.size() - G.col_off;
+      for (int c0 = 0; c0 < G.ncol; c0 += 64) {
+        int kmin = 1;
+        for (int c = c0; c < std::min(c0 + 64, G.ncol); c++)
+          kmin = std::min(kmin, T.lo[2] + (geo_cols[G.col_off + c] >> 16));
+        geo_kmin.push_back(kmin);
+      }
+      it = class_of.emplace(key, (int)geo.size()).first;
+      geo.push_back(G);
+    }
+    T.geo = it->second;
+  }
+  tl->ngeo = (int)geo.size();
+
   // --- per-level ranges of the orthorhombic tasks ---
   tl->olev.assign(nlevels, OrthoLevel{});
   tl->lp0s.resize(ntasks);
@@ -604,7 +691,13 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     if (o.end == 0) o.begin = s;
     o.end = s + 1;
     for (int d = 0; d < 3; d++) o.max_ext = std::max(o.max_ext, 2 - 2 * T.lo[d]);
+    if (T.geo < 0)
+      o.classed = false;
+    else
+      o.points += geo[T.geo].npts;
   }
+  for (auto &o : tl->olev)
+    if (o.end > o.begin && o.classed) tl->class_points += o.points;
   for (auto &o : tl->olev)
     for (int v = 0; v < kLp0Slots; v++) {
       int s = o.begin;
@@ -630,6 +723,9 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   upload(tl->d_sphi, sphi_all);
   upload(tl->d_fptr, fptr);
   upload(tl->d_fent, fent);
+  upload(tl->d_geo, geo);
+  upload(tl->d_geo_cols, geo_cols);
+  upload(tl->d_geo_kmin, geo_kmin);
   tl->d_levels.get(sizeof(Level) * nlevels);
   if (!tl->h_levels) HIP_CHECK(hipHostMalloc((void **)&tl->h_levels, sizeof(Level) * 64));
   if (nlevels > 64) fatal("at most 64 grid levels are supported");
@@ -720,32 +816,41 @@ int num_cus() {
 enum LaunchMode { kCollocate = 0, kIntegrate = 1, kCount = 2 };
 
 // LDS plan of a wave-per-task launch over orthorhombic tasks of one level
-// with lp <= `lp`.  Collocation keeps the whole level grid in LDS when it
-// fits next to at least four wave regions.
-WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[4], int mode) {
+// with lp <= `lp`.  Setup/finish kernels hold dense cxyz and {cab, alpha} per
+// wave; point kernels hold the per-axis tables, and collocation keeps the
+// whole level grid in LDS when it fits next to at least four wave regions.
+enum PlanKind { kPlanSetup, kPlanPoints, kPlanWindow };
+
+WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[4], int kind) {
   const OrthoLevel &o = tl->olev[level];
   WavePlan w{};
   w.level = level;
   w.np = lp + 1;
   w.ext = o.max_ext;
-  w.i0n = o.max_ext / 2;
-  // cab [ncoset(la)][ncoset(lb)] + alpha [3][lb+1][la+1][lp+1] for la + lb <= lp
-  int n_setup = 0;
-  for (int la = 0; la <= tl->max_la + ld[0]; la++)
-    for (int lb = 0; lb <= tl->max_lb + ld[2] && la + lb <= lp; lb++)
-      n_setup = std::max(n_setup, ncoset(la) * ncoset(lb) + 3 * (la + 1) * (lb + 1) * w.np);
-  w.o_cxyz = 0;
-  w.o_u = w.np * w.np * w.np;
-  const int nd = w.o_u + std::max(n_setup, 3 * w.ext * w.np);
-  w.o_i0 = 8 * nd;
-  w.o_col = w.o_i0 + ((w.i0n * w.i0n + 3) & ~3);
-  w.wave_bytes = (w.o_col + 4 * (w.ext + 2) + 15) & ~15;
+  int nd;
+  if (kind == kPlanSetup) {
+    // cab [ncoset(la)][ncoset(lb)] + alpha [3][lb+1][la+1][lp+1] for la + lb <= lp
+    int n_setup = 0;
+    for (int la = 0; la <= tl->max_la + ld[0]; la++)
+      for (int lb = 0; lb <= tl->max_lb + ld[2] && la + lb <= lp; lb++)
+        n_setup = std::max(n_setup, ncoset(la) * ncoset(lb) + 3 * (la + 1) * (lb + 1) * w.np);
+    w.o_cxyz = 0;
+    w.o_u = w.np * w.np * w.np;
+    nd = w.o_u + n_setup;
+  } else {
+    // packed cxyz of the task (collocate) + per-axis tables
+    w.o_cxyz = 0;
+    w.o_u = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+    nd = w.o_u + 3 * w.ext * w.np;
+  }
+  w.wave_bytes = (8 * nd + 15) & ~15;
   const int limit = lds_limit();
-  if (mode == kCollocate) {
+  if (kind == kPlanWindow) {
     const long long npts = (long long)tl->npts_local[3 * level] * tl->npts_local[3 * level + 1] *
                            tl->npts_local[3 * level + 2];
-    for (int nw : {8, 6, 4})
-      if (8 * npts + (long long)nw * w.wave_bytes <= limit) {
+    const int most = lp <= 4 ? 12 : 8;  // launch bounds of the window kernels
+    for (int nw : {12, 10, 8, 6, 4})
+      if (nw <= most && 8 * npts + (long long)nw * w.wave_bytes <= limit) {
         w.window = (int)npts;
         w.nwaves = nw;
         break;
@@ -766,8 +871,16 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
   return w;
 }
 
-// Orthorhombic tasks, level by level: wave-per-task kernels for lp <= 6 and
-// cube extents <= kWaveMaxExt, the per-task kernels for the rest.
+// Doubles per task of the packed cxyz buffer for a call with lp shift sh.
+int cxyz_stride(const TaskListImpl *tl, int sh) {
+  const int lp = std::min(6, tl->max_lp + sh);
+  return (lp + 1) * (lp + 2) * (lp + 3) / 6;
+}
+
+// Orthorhombic tasks, level by level: wave-per-task kernels for the levels
+// whose tasks all have a geometry class, for lp <= 6; the per-task kernels
+// for the rest.  Collocate: cab -> cxyz pass, then the point kernels;
+// integrate: the point kernels, then the cxyz -> hab/forces pass.
 void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   const int sh = P.ld[0] + P.ld[2];
   for (int l = 0; l < tl->nlevels; l++) {
@@ -775,33 +888,42 @@ void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode
     if (o.end <= o.begin) continue;
     auto first = [&](int v) { return o.lp_start[std::min(std::max(v, 0), kLp0Slots - 1)]; };
     int c1 = first(5 - sh), c2 = first(7 - sh);  // lp <= 4 | lp 5..6 | larger
-    if (mode == kCount) c1 = c2 = o.end;          // counting ignores lp
-    if (!wave_kernels() || o.max_ext > kWaveMaxExt) c1 = c2 = o.begin;
-    const int cuts[3][2] = {{o.begin, c1}, {c1, c2}, {c2, o.end}};
-    for (int r = 0; r < 2; r++) {
-      const int a = cuts[r][0], b = cuts[r][1];
-      if (b <= a) continue;
-      const int lpb = r == 0 ? 4 : 6;
-      const int lp = mode == kCount ? 0 : std::min(lpb, tl->lp0s[b - 1] + sh);
+    if (!wave_kernels() || !o.classed) c1 = c2 = o.begin;
+    if (c2 > o.begin) {
+      const int lp_all = std::min(6, tl->lp0s[c2 - 1] + sh);
       PointParams Q = P;
-      Q.task_base = a;
-      Q.ntasks = b - a;
-      Q.wp = make_wave_plan(tl, l, lp, P.ld, mode);
-      if (mode == kIntegrate) {
-        HIP_CHECK(launch_integrate_wave(Q, lpb, s));
-      } else {
-        int blocks = (Q.ntasks + Q.wp.nwaves - 1) / Q.wp.nwaves;
-        if (Q.wp.window) {
-          const int per_cu = std::max(1, lds_limit() / Q.wp.bytes);
-          blocks = std::min(blocks, num_cus() * per_cu);
+      Q.task_base = o.begin;
+      Q.ntasks = c2 - o.begin;
+      Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanSetup);
+      if (mode == kCollocate) HIP_CHECK(launch_cxyz_wave(Q, s));
+      const int cuts[2][2] = {{o.begin, c1}, {c1, c2}};
+      for (int r = 0; r < 2; r++) {
+        const int a = cuts[r][0], b = cuts[r][1];
+        if (b <= a) continue;
+        const int lpb = r == 0 ? 4 : 6;
+        const int lp = std::min(lpb, tl->lp0s[b - 1] + sh);
+        PointParams R = P;
+        R.task_base = a;
+        R.ntasks = b - a;
+        if (mode == kIntegrate) {
+          R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanPoints);
+          HIP_CHECK(launch_integrate_wave(R, lpb, s));
+        } else {
+          R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanWindow);
+          int blocks = (R.ntasks + R.wp.nwaves - 1) / R.wp.nwaves;
+          if (R.wp.window) {
+            const int per_cu = std::max(1, lds_limit() / R.wp.bytes);
+            blocks = std::min(blocks, num_cus() * per_cu);
+          }
+          HIP_CHECK(launch_collocate_wave(R, lpb, blocks, s));
         }
-        HIP_CHECK(launch_collocate_wave(Q, lpb, blocks, s));
       }
+      if (mode == kIntegrate) HIP_CHECK(launch_finish_wave(Q, s));
     }
-    if (o.end > cuts[2][0]) {
+    if (o.end > c2) {
       PointParams Q = P;
-      Q.task_base = cuts[2][0];
-      Q.ntasks = o.end - cuts[2][0];
+      Q.task_base = c2;
+      Q.ntasks = o.end - c2;
       HIP_CHECK(mode == kIntegrate ? launch_integrate(Q, true, s) : launch_collocate(Q, true, s));
     }
   }
@@ -810,12 +932,34 @@ void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode
 // Launches the orthorhombic and the general task ranges (tasks are sorted
 // by class: orthorhombic, general, skipped).
 void launch_ranges(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
-  if (tl->n_ortho > 0) launch_ortho(tl, P, s, mode);
+  if (mode == kCount) {
+    // tasks with a geometry class are counted on the host
+    for (int l = 0; l < tl->nlevels; l++) {
+      const OrthoLevel &o = tl->olev[l];
+      if (o.end <= o.begin || o.classed) continue;
+      PointParams Q = P;
+      Q.task_base = o.begin;
+      Q.ntasks = o.end - o.begin;
+      HIP_CHECK(launch_collocate(Q, true, s));
+    }
+  } else if (tl->n_ortho > 0) {
+    launch_ortho(tl, P, s, mode);
+  }
   if (tl->n_general > 0) {
     P.task_base = (int)tl->n_ortho;
     P.ntasks = (int)tl->n_general;
     HIP_CHECK(mode == kIntegrate ? launch_integrate(P, false, s) : launch_collocate(P, false, s));
   }
+}
+
+// Geometry classes and the per-task cxyz buffer of the wave-per-task kernels.
+void set_wave_buffers(TaskListImpl *tl, PointParams &P) {
+  P.geo = (const GeoClass *)tl->d_geo.p;
+  P.geo_cols = (const int *)tl->d_geo_cols.p;
+  P.geo_kmin = (const int *)tl->d_geo_kmin.p;
+  P.cxyz_stride = cxyz_stride(tl, P.ld[0] + P.ld[2]);
+  P.cxyz_all = (double *)tl->d_cxyz.get(sizeof(double) * (size_t)P.cxyz_stride *
+                                        (size_t)std::max(tl->ntasks, 1));
 }
 
 void set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s) {
@@ -876,6 +1020,7 @@ void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const 
   P.func = func;
   for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
   P.plan = make_plan(tl, ld, false);
+  set_wave_buffers(tl, P);
   {
     TimedLaunch tm(0, s);
     launch_ranges(tl, P, s, kCollocate);
@@ -914,6 +1059,7 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   P.do_forces = do_f;
   P.do_virial = do_v;
   P.plan = make_plan(tl, ld, true);
+  set_wave_buffers(tl, P);
   if (do_f && tl->n_skip > 0) {  // skipped tasks contribute no forces
     const long long first = (long long)(tl->n_ortho + tl->n_general) * kTaskOutStride;
     HIP_CHECK(launch_fill((double *)tl->d_task_out.p + first,
@@ -1420,8 +1566,14 @@ long long grid_mi355x_task_points(const grid_task_list *task_list) {
   HIP_CHECK(hipMemcpyAsync(out.data(), tl->d_task_out.p, sizeof(double) * out.size(),
                            hipMemcpyDeviceToHost, s));
   sync_stream(s);
-  long long total = 0;
-  for (int t = 0; t < tl->ntasks; t++) total += (long long)out[(size_t)t * kTaskOutStride];
+  // classed levels from the geometry classes, the rest from the kernel counts
+  long long total = tl->class_points;
+  auto add = [&](long long a, long long b) {
+    for (long long t = a; t < b; t++) total += (long long)out[(size_t)t * kTaskOutStride];
+  };
+  for (const OrthoLevel &o : tl->olev)
+    if (o.end > o.begin && !o.classed) add(o.begin, o.end);
+  add(tl->n_ortho, tl->n_ortho + tl->n_general);
   tl->points = total;
   return total;
 }

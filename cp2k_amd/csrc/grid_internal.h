@@ -49,7 +49,19 @@ struct Task {
   int level, flags, group;
   int o1, o2;          // Cartesian offsets of this pgf pair in the group block
   int iatom, jatom;    // zero based
-  int pad_;
+  int geo;             // orthorhombic: geometry class (GeoClass index)
+};
+
+// Truncation region shared by the orthorhombic tasks of one level with the
+// same discretised radius (it does not depend on the task's centre): the
+// columns (i, j) of the central plane in j-major order, each packed as
+// (i - lo_x) | (j - lo_y) << 8 | (kl - lo_z) << 16 with planes kl .. 1 - kl,
+// and per group of 64 columns the smallest kl.
+struct GeoClass {
+  int col_off, ncol;   // into the column array
+  int grp_off;         // into the group minimum array
+  int lo[3];
+  long long npts;      // points of one task of the class
 };
 
 // One (matrix block, iset, jset) sub-block: the unit of (de)contraction.
@@ -90,17 +102,15 @@ inline int lpb_bucket(int lp) { return lp <= 4 ? 4 : lp <= 8 ? 8 : lp <= 12 ? 12
 // a workgroup owns an LDS region of wave_bytes; collocation into a level
 // whose whole grid fits into LDS keeps that grid ("window", in doubles) at
 // the start of LDS and adds it to HBM once per workgroup.
-constexpr int kWaveMaxExt = 64;  // cube extent per axis handled by one wave
+constexpr int kWaveMaxExt = 256;  // cube extent per axis (8-bit packed columns)
 struct WavePlan {
   int level;       // grid level of the launch
   int np;          // polynomial stride (max lp + 1)
   int ext;         // per-axis table stride (max cube extent 2 - 2 lo)
-  int i0n;         // stride of the row-start table (max 1 - lo)
   int nwaves;      // waves per workgroup
   int wave_bytes;  // LDS bytes per wave
   int window;      // doubles of the LDS level window (0: accumulate in HBM)
-  int o_cxyz, o_u; // doubles: cxyz [np^3]; union of {cab, alpha} and tables [3][ext][np]
-  int o_i0, o_col; // bytes: int8 row starts [i0n][i0n]; int column starts [ext + 1]
+  int o_cxyz, o_u; // doubles: cxyz [np^3]; {cab, alpha} (setup/finish) or tables [3][ext][np]
   int bytes;       // dynamic LDS per workgroup
 };
 
@@ -120,6 +130,12 @@ struct PointParams {
   int count_only;         // count task points into task_out[t*15]
   LdsPlan plan;
   WavePlan wp;
+  // wave-per-task kernels
+  const GeoClass *geo;
+  const int *geo_cols;
+  const int *geo_kmin;
+  double *cxyz_all;       // per task: cxyz packed (lz, ly, lx), lx fastest, lx+ly+lz <= lp
+  int cxyz_stride;        // doubles per task
 };
 
 }  // namespace gm

@@ -1,27 +1,31 @@
 // Wave-per-task kernels for orthorhombic tasks (gfx950, wave64).
 //
-// One wave processes one task at a time, with no workgroup barriers: the
-// task record is read through __restrict__ pointers into scalar registers
-// and every per-task table lives in the wave's own LDS region:
-//   cab -> cxyz about rp (collocate; src/grid/ref/grid_ref_collint.h:877-910)
-//   tab[d][c][l] = (c h_d - roff_d)^l exp(-zetp (c h_d - roff_d)^2)
-//   i0[|k'|][|j'|] (int8): first x of row (j, k) of the truncation region
-//   column starts of the central plane
-// The points are walked column by column.  Lanes take consecutive (i, j)
-// columns of the central plane (x fastest, so a wave's loads and adds are
-// contiguous along x) and step through the planes k together.  Membership
-// repeats the reference's bounds (grid_ref_collint.h:139-167): plane k keeps
-// the rows with j' >= js(k) and row (j, k) the points with i' >= i0(j, k),
-// where i' = i for i <= 0 and 1 - i otherwise.  Both bounds only grow with
-// |k'| (every operation in them is monotone), so the planes of a column form
-// the interval [kl, 1 - kl]; kl is found by bisection in the i0 table.
+// One wave processes one task at a time, with no workgroup barriers on the
+// task path; task records are read through __restrict__ pointers into scalar
+// registers.  A collocate or integrate call runs three kernels:
+//
+//   cxyz_wave_kernel      cab (prepare_pab of the decontracted density) ->
+//                         cxyz about rp (src/grid/ref/grid_ref_collint.h
+//                         :877-910), stored per task in HBM (collocate)
+//   *_points kernels      the truncation region, point by point
+//   finish_wave_kernel    cxyz -> cab -> hab, forces, virial (integrate;
+//                         grid_ref_integrate.c:132-171)
+//
+// Point walk.  The truncation region of a task depends only on its level and
+// discretised radius, so the host builds it once per such geometry class:
+// the columns (i, j) of the central plane, each with its plane interval
+// [kl, 1 - kl], computed with the reference's floating-point bounds
+// (grid_ref_collint.h:139-167).  Lanes take consecutive columns (x fastest,
+// so loads and adds are contiguous along x) and step through the planes k
+// together, reading per-axis tables (c h - roff)^l exp(-zetp (c h - roff)^2)
+// from the wave's LDS region.
 //
 // Collocate: per column D[lz] = sum_{lx,ly} cxyz px py, then per point
 // (lp + 1) fused multiply-adds and one atomic add, either into an LDS copy of
-// the whole level grid shared by the workgroup (flushed to HBM once at the
-// end) or straight into HBM.  Integrate: per point (lp + 1) multiply-adds of
-// the z moments, per column the outer product with px, py into lane-private
-// cxyz accumulators, and one cross-lane reduction per task.
+// the whole level grid shared by the workgroup (added to HBM once at the end)
+// or straight into HBM.  Integrate: per point (lp + 1) multiply-adds of the z
+// moments, per column their outer product with px, py into lane-private cxyz
+// accumulators, one cross-lane reduction per task.
 //
 // Compiled with -ffp-contract=off; fused multiply-adds are explicit fma().
 
@@ -60,12 +64,6 @@ __device__ __forceinline__ double wave_total(double v) {
   return v;
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
 __device__ __forceinline__ int pm(int a, int m) {
   const int r = a % m;
   return r < 0 ? r + m : r;
@@ -73,11 +71,25 @@ __device__ __forceinline__ int pm(int a, int m) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// a mod n for a within a few periods of [0, n).
+__device__ __forceinline__ int wrap(int a, int n) {
+  while (a < 0) a += n;
+  while (a >= n) a -= n;
+  return a;
+}
+
+// Packed index of (lx, ly, lz), lx + ly + lz <= lp: lz slowest, lx fastest.
+__device__ __forceinline__ int packed_index(int lx, int ly, int lz, int lp) {
+  int off = 0;
+  for (int z = 0; z < lz; z++) off += (lp - z + 1) * (lp - z + 2) / 2;
+  for (int y = 0; y < ly; y++) off += lp - lz - y + 1;
+  return off + lx;
+}
+
 // Per-wave LDS region.
 struct WaveLds {
-  double *cxyz, *u;  // u: {cab, alpha} during setup/finish, tables while walking
-  signed char *i0;
-  int *col;
+  double *cxyz;  // dense [np][np][np] (setup, finish) or packed (collocate points)
+  double *u;     // {cab, alpha} (setup/finish) or per-axis tables (point kernels)
 };
 
 __device__ __forceinline__ WaveLds wave_lds(double *lds, const WavePlan &wp, int wave) {
@@ -85,51 +97,7 @@ __device__ __forceinline__ WaveLds wave_lds(double *lds, const WavePlan &wp, int
   WaveLds w;
   w.cxyz = reinterpret_cast<double *>(b) + wp.o_cxyz;
   w.u = reinterpret_cast<double *>(b) + wp.o_u;
-  w.i0 = reinterpret_cast<signed char *>(b + wp.o_i0);
-  w.col = reinterpret_cast<int *>(b + wp.o_col);
   return w;
-}
-
-// Row-start table and the column starts of the central plane (k' = 0, the
-// plane that contains every column).  Returns the number of columns.
-__device__ int wave_geometry(const Task &T, const Layout &lay, const WavePlan &wp,
-                             const WaveLds &W, int lane) {
-  const int nkz = 1 - T.lo[2], njy = 1 - T.lo[1];
-  const double dz = lay.dh[2][2], dy = lay.dh[1][1];
-  const double dyinv = lay.dh_inv[1][1], dxinv = lay.dh_inv[0][0];
-  for (int e = lane; e < nkz * njy; e += 64) {
-    const int kz = e / njy, jy = e - kz * njy;
-    const int k1 = T.lo[2] + kz, j1 = T.lo[1] + jy;
-    const double kr = k1 * dz;
-    const double krem = T.disr * T.disr - kr * kr;
-    int i0 = 127;
-    if (j1 >= ortho_start(krem, dyinv)) {
-      const double jr = j1 * dy;
-      const double jrem = krem - jr * jr;
-      i0 = ortho_start(jrem, dxinv);
-    }
-    W.i0[kz * wp.i0n + jy] = (signed char)i0;
-  }
-  wave_sync();
-  const int ny_ext = 2 * njy;  // <= 64, checked on the host
-  int len = 0;
-  if (lane < ny_ext) {
-    const int j = T.lo[1] + lane;
-    const int j1 = j <= 0 ? j : 1 - j;
-    const int i0 = W.i0[(nkz - 1) * wp.i0n + (j1 - T.lo[1])];
-    len = i0 == 127 ? 0 : 2 - 2 * i0;
-  }
-  int incl = len;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += y;
-  }
-  if (lane < ny_ext) W.col[lane + 1] = incl;
-  if (lane == 0) W.col[0] = 0;
-  const int ncol = __shfl(incl, ny_ext - 1, 64);
-  wave_sync();
-  return ncol;
 }
 
 // Per-axis tables, tab[(d * ext + c - lo_d) * np + l].
@@ -154,43 +122,44 @@ __device__ void wave_tables(const Task &T, const Layout &lay, int lp, const Wave
   wave_sync();
 }
 
-// Column c of the central plane -> cube indices, plane interval, grid base.
+// Column c0 + lane of the task's geometry class.
 struct Column {
   int i, j;  // cube indices
   int kl;    // planes kl .. 1 - kl; kl = 1 marks a lane without a column
   int base;  // y * nx + x of the column in the local grid
 };
 
-__device__ __forceinline__ Column decode_column(const Task &T, const Layout &lay,
-                                                const WavePlan &wp, const WaveLds &W,
-                                                int ncol, int c) {
+__device__ __forceinline__ Column load_column(const PointParams &P, const GeoClass &G,
+                                              const Layout &lay, const int cw[3], int c) {
   Column C{0, 0, 1, 0};
-  if (c >= ncol) return C;
-  const int nkz = 1 - T.lo[2], ny_ext = 2 - 2 * T.lo[1];
-  int lo = 0, hi = ny_ext;  // col[ny_ext] = ncol > c
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (W.col[mid] <= c) lo = mid; else hi = mid;
+  if (c >= G.ncol) return C;
+  const int rec = P.geo_cols[G.col_off + c];
+  C.i = G.lo[0] + (rec & 255);
+  C.j = G.lo[1] + ((rec >> 8) & 255);
+  const int gx = wrap(cw[0] + C.i, lay.npts_global[0]);
+  const int gy = wrap(cw[1] + C.j, lay.npts_global[1]);
+  // a cube that fits the local grid (checked on the host) never fails this
+  if (gx < lay.npts_local[0] && gy < lay.npts_local[1]) {
+    C.kl = G.lo[2] + (rec >> 16);
+    C.base = gy * lay.npts_local[0] + gx;
   }
-  const int j = T.lo[1] + lo;
-  const int jy = (j <= 0 ? j : 1 - j) - T.lo[1];
-  const int i = W.i0[(nkz - 1) * wp.i0n + jy] + (c - W.col[lo]);
-  const int i1 = i <= 0 ? i : 1 - i;
-  int a = 0, b = nkz - 1;  // the central plane always holds the column
-  while (a < b) {
-    const int m = (a + b) >> 1;
-    if (W.i0[m * wp.i0n + jy] <= i1) b = m; else a = m + 1;
-  }
-  C.i = i;
-  C.j = j;
-  C.kl = T.lo[2] + a;
-  const int gx = pm(T.center[0] + i - lay.shift_local[0], lay.npts_global[0]);
-  const int gy = pm(T.center[1] + j - lay.shift_local[1], lay.npts_global[1]);
-  C.base = gy * lay.npts_local[0] + gx;
   return C;
 }
 
-// cab (prepare_pab of the decontracted density) -> cxyz, in the wave's LDS.
+template <int LP>
+__device__ __forceinline__ void load_pxy(const Task &T, const WavePlan &wp, const double *tab,
+                                         const Column &C, double px[LP + 1], double py[LP + 1]) {
+  const int np = wp.np;
+  const double *rx = tab + clampi(C.i - T.lo[0], 0, wp.ext - 1) * np;
+  const double *ry = tab + (wp.ext + clampi(C.j - T.lo[1], 0, wp.ext - 1)) * np;
+#pragma unroll
+  for (int l = 0; l <= LP; l++) {
+    px[l] = rx[l];
+    py[l] = ry[l];
+  }
+}
+
+// cab (prepare_pab of the decontracted density) -> dense cxyz in the wave's LDS.
 __device__ void wave_cxyz(const PointParams &P, const Task &T, const Group &G, int la, int lb,
                           int la_min, int lb_min, int NP, const WaveLds &W, int lane) {
   const int n1 = ncoset(la), n2 = ncoset(lb);
@@ -232,33 +201,28 @@ __device__ void wave_cxyz(const PointParams &P, const Task &T, const Group &G, i
   wave_sync();
 }
 
-template <int LP>
-__device__ __forceinline__ void load_pxy(const Task &T, const WavePlan &wp, const double *tab,
-                                         const Column &C, double px[LP + 1], double py[LP + 1]) {
-  const int np = wp.np;
-  const double *rx = tab + clampi(C.i - T.lo[0], 0, wp.ext - 1) * np;
-  const double *ry = tab + (wp.ext + clampi(C.j - T.lo[1], 0, wp.ext - 1)) * np;
-#pragma unroll
-  for (int l = 0; l <= LP; l++) {
-    px[l] = rx[l];
-    py[l] = ry[l];
-  }
+__device__ __forceinline__ void axis_offsets(const Task &T, const Layout &lay, int cw[3]) {
+  cw[0] = pm(T.center[0] - lay.shift_local[0], lay.npts_global[0]);
+  cw[1] = pm(T.center[1] - lay.shift_local[1], lay.npts_global[1]);
+  cw[2] = pm(T.center[2] - lay.shift_local[2], lay.npts_global[2]);
 }
 
-// Collocate the task's points; dst is the LDS window or the HBM level grid
-// (npts points either way).
+// Collocate the task's points; dst is the LDS window or the HBM level grid.
 template <int LP, typename Dst>
-__device__ void collocate_columns(const Task &T, const Layout &lay, const WavePlan &wp,
-                                  const WaveLds &W, int ncol, Dst *dst, int npts, int lane) {
+__device__ __forceinline__ void collocate_columns(const PointParams &P, const Task &T, const GeoClass &G,
+                                  const Layout &lay, const WavePlan &wp, const double *tab,
+                                  const double *cx, Dst *dst, int lane) {
   const int NP = wp.np;
-  const double *tab = W.u;
   const double *tabz = tab + 2 * wp.ext * NP;
   const int nxy = lay.npts_local[0] * lay.npts_local[1];
-  const int ngz = lay.npts_global[2];
-  for (int c0 = 0; c0 < ncol; c0 += 64) {
-    const Column C = decode_column(T, lay, wp, W, ncol, c0 + lane);
+  const int ngz = lay.npts_global[2], nz = lay.npts_local[2];
+  int cw[3];
+  axis_offsets(T, lay, cw);
+  for (int c0 = 0; c0 < G.ncol; c0 += 64) {
+    const Column C = load_column(P, G, lay, cw, c0 + lane);
     double px[LP + 1], py[LP + 1], D[LP + 1];
     load_pxy<LP>(T, wp, tab, C, px, py);
+    int e = 0;
 #pragma unroll
     for (int lz = 0; lz <= LP; lz++) {
       double s = 0.0;
@@ -266,49 +230,51 @@ __device__ void collocate_columns(const Task &T, const Layout &lay, const WavePl
       for (int ly = 0; ly <= LP - lz; ly++) {
         double t = 0.0;
 #pragma unroll
-        for (int lx = 0; lx <= LP - lz - ly; lx++)
-          t = fma(W.cxyz[(lz * NP + ly) * NP + lx], px[lx], t);
+        for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fma(cx[e], px[lx], t);
         s = fma(t, py[ly], s);
       }
       D[lz] = s;
     }
-    const int kmin = wave_min(C.kl);
-    int gz = pm(T.center[2] + kmin - lay.shift_local[2], ngz);
+    const int kmin = P.geo_kmin[G.grp_off + (c0 >> 6)];
+    const unsigned span = (unsigned)(2 - 2 * C.kl);  // 0 for an idle lane
+    int gz = wrap(cw[2] + kmin, ngz);
+#pragma unroll 2
     for (int k = kmin; k <= 1 - kmin; k++) {
       const double *pz = tabz + (k - T.lo[2]) * NP;
       double v = 0.0;
 #pragma unroll
       for (int lz = 0; lz <= LP; lz++) v = fma(D[lz], pz[lz], v);
-      const int idx = gz * nxy + C.base;
-      if (k >= C.kl && k <= 1 - C.kl && idx < npts) atomicAdd(&dst[idx], v);
+      if (gz < nz && (unsigned)(k - C.kl) < span) atomicAdd(&dst[gz * nxy + C.base], v);
       gz = (gz + 1 == ngz) ? 0 : gz + 1;
     }
   }
 }
 
-// Integrate: lane-private cxyz accumulators acc[(lz, ly, lx)], lx fastest.
+// Integrate the task's points into lane-private accumulators acc[(lz, ly, lx)].
 template <int LP>
-__device__ void integrate_columns(const Task &T, const Layout &lay, const WavePlan &wp,
-                                  const WaveLds &W, int ncol, const double *grid, int npts,
-                                  double *acc, int lane) {
+__device__ __forceinline__ void integrate_columns(const PointParams &P, const Task &T, const GeoClass &G,
+                                  const Layout &lay, const WavePlan &wp, const double *tab,
+                                  const double *grid, double *acc, int lane) {
   const int NP = wp.np;
-  const double *tab = W.u;
   const double *tabz = tab + 2 * wp.ext * NP;
   const int nxy = lay.npts_local[0] * lay.npts_local[1];
-  const int ngz = lay.npts_global[2];
-  for (int c0 = 0; c0 < ncol; c0 += 64) {
-    const Column C = decode_column(T, lay, wp, W, ncol, c0 + lane);
+  const int ngz = lay.npts_global[2], nz = lay.npts_local[2];
+  int cw[3];
+  axis_offsets(T, lay, cw);
+  for (int c0 = 0; c0 < G.ncol; c0 += 64) {
+    const Column C = load_column(P, G, lay, cw, c0 + lane);
     double px[LP + 1], py[LP + 1], M[LP + 1];
     load_pxy<LP>(T, wp, tab, C, px, py);
 #pragma unroll
     for (int l = 0; l <= LP; l++) M[l] = 0.0;
-    const int kmin = wave_min(C.kl);
-    int gz = pm(T.center[2] + kmin - lay.shift_local[2], ngz);
+    const int kmin = P.geo_kmin[G.grp_off + (c0 >> 6)];
+    const unsigned span = (unsigned)(2 - 2 * C.kl);
+    int gz = wrap(cw[2] + kmin, ngz);
+#pragma unroll 2
     for (int k = kmin; k <= 1 - kmin; k++) {
       const double *pz = tabz + (k - T.lo[2]) * NP;
-      const int idx = gz * nxy + C.base;
       double g = 0.0;
-      if (k >= C.kl && k <= 1 - C.kl && idx < npts) g = grid[idx];
+      if (gz < nz && (unsigned)(k - C.kl) < span) g = grid[gz * nxy + C.base];
 #pragma unroll
       for (int l = 0; l <= LP; l++) M[l] = fma(g, pz[l], M[l]);
       gz = (gz + 1 == ngz) ? 0 : gz + 1;
@@ -326,28 +292,19 @@ __device__ void integrate_columns(const Task &T, const Layout &lay, const WavePl
 }
 
 template <int LP>
-__device__ void integrate_moments(const Task &T, const Layout &lay, const WavePlan &wp,
-                                  const WaveLds &W, int ncol, const double *grid, int npts,
-                                  int lane) {
+__device__ __forceinline__ void integrate_task(const PointParams &P, const Task &T, const GeoClass &G,
+                               const Layout &lay, const WavePlan &wp, const double *tab,
+                               const double *grid, double *out, int lane) {
   constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
   double acc[NC];
 #pragma unroll
   for (int e = 0; e < NC; e++) acc[e] = 0.0;
-  integrate_columns<LP>(T, lay, wp, W, ncol, grid, npts, acc, lane);
-  const int NP = wp.np;
-  for (int e = lane; e < NP * NP * NP; e += 64) W.cxyz[e] = 0.0;
-  wave_sync();
-  int e = 0;
+  integrate_columns<LP>(P, T, G, lay, wp, tab, grid, acc, lane);
 #pragma unroll
-  for (int lz = 0; lz <= LP; lz++)
-#pragma unroll
-    for (int ly = 0; ly <= LP - lz; ly++)
-#pragma unroll
-      for (int lx = 0; lx <= LP - lz - ly; lx++, e++) {
-        const double s = wave_total(acc[e]);
-        if (lane == 63) W.cxyz[(lz * NP + ly) * NP + lx] = s;
-      }
-  wave_sync();
+  for (int e = 0; e < NC; e++) {
+    const double s = wave_total(acc[e]);
+    if (lane == 63) out[e] = s;
+  }
 }
 
 // cxyz -> cab -> hab (Cartesian, atomics into the group's slice), forces and
@@ -415,19 +372,54 @@ __device__ void wave_finish(const PointParams &P, const Task &T, const Group &G,
       if (lane == 63) out[c] = scale * s;
     }
   }
-  wave_sync();
+}
+
+struct TaskL {
+  int la, lb, la_min, lb_min, lp;
+};
+
+__device__ __forceinline__ TaskL task_l(const PointParams &P, const Task &T) {
+  TaskL l;
+  l.la = T.la_max + P.ld[0];
+  l.lb = T.lb_max + P.ld[2];
+  l.la_min = max(T.la_min + P.ld[1], 0);
+  l.lb_min = max(T.lb_min + P.ld[3], 0);
+  l.lp = l.la + l.lb;
+  return l;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Kernels.  Launch ranges hold orthorhombic tasks of level wp.level with
-// lp <= LPB and cube extents <= kWaveMaxExt.
+// Kernels.  Launch ranges hold orthorhombic tasks of level wp.level that
+// belong to a geometry class, with lp <= LPB (point kernels).
 // ---------------------------------------------------------------------------
+
+// cab -> cxyz of one task per wave, stored packed at cxyz_all[task * stride].
+__global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__ tasks,
+                                                        const Group *__restrict__ groups,
+                                                        PointParams P) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= P.ntasks) return;
+  const long long task = P.task_base + t;
+  const Task &T = tasks[task];
+  const TaskL l = task_l(P, T);
+  const WaveLds W = wave_lds(lds, P.wp, wave);
+  const int NP = P.wp.np;
+  wave_cxyz(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, NP, W, lane);
+  double *out = P.cxyz_all + task * P.cxyz_stride;
+  for (int e = lane; e < NP * NP * NP; e += 64) {
+    const int lz = e / (NP * NP), ly = (e / NP) % NP, lx = e % NP;
+    if (lx + ly + lz <= l.lp) out[packed_index(lx, ly, lz, l.lp)] = W.cxyz[e];
+  }
+}
+
 template <int LPB, bool WINDOW>
-__global__ __launch_bounds__(512) void collocate_wave_kernel(const Task *__restrict__ tasks,
+__global__ __launch_bounds__(WINDOW ? (LPB <= 4 ? 768 : 512) : 256) void collocate_wave_kernel(const Task *__restrict__ tasks,
                                                              const Level *__restrict__ levels,
-                                                             const Group *__restrict__ groups,
                                                              PointParams P) {
   extern __shared__ double lds[];
   const WavePlan &wp = P.wp;
@@ -436,42 +428,31 @@ __global__ __launch_bounds__(512) void collocate_wave_kernel(const Task *__restr
   const int nw = blockDim.x >> 6;
   const Level &L = levels[wp.level];
   const Layout &lay = L.lay;
-  const int npts = (int)L.npts;
   if (WINDOW) {
     for (int e = threadIdx.x; e < wp.window; e += blockDim.x) lds[e] = 0.0;
     __syncthreads();
   }
   const WaveLds W = wave_lds(lds, wp, wave);
+  double *dst = WINDOW ? lds : L.grid;
   const long long stride = (long long)gridDim.x * nw;
   for (long long t = (long long)blockIdx.x * nw + wave; t < P.ntasks; t += stride) {
     const long long task = P.task_base + t;
     const Task &T = tasks[task];
-    const int la = T.la_max + P.ld[0], lb = T.lb_max + P.ld[2];
-    const int la_min = max(T.la_min + P.ld[1], 0), lb_min = max(T.lb_min + P.ld[3], 0);
-    const int lp = la + lb;
-    if (!P.count_only) wave_cxyz(P, T, groups[T.group], la, lb, la_min, lb_min, wp.np, W, lane);
-    const int ncol = wave_geometry(T, lay, wp, W, lane);
-    if (P.count_only) {
-      double cnt = 0.0;
-      for (int c0 = 0; c0 < ncol; c0 += 64) {
-        const Column C = decode_column(T, lay, wp, W, ncol, c0 + lane);
-        cnt += (double)(C.kl <= 0 ? 2 - 2 * C.kl : 0);
-      }
-      cnt = wave_total(cnt);
-      if (lane == 63) P.task_out[task * kTaskOutStride] = cnt;
-      wave_sync();
-      continue;
-    }
-    wave_tables(T, lay, lp, wp, W.u, lane);
-    double *dst = WINDOW ? lds : L.grid;
+    const GeoClass &G = P.geo[T.geo];
+    const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+    const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+    const double *src = P.cxyz_all + task * P.cxyz_stride;
+    for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
+    wave_tables(T, lay, lp, wp, W.u, lane);  // ends with wave_sync
+    const double *cx = W.cxyz;
     switch (lp) {
-      case 0: collocate_columns<0>(T, lay, wp, W, ncol, dst, npts, lane); break;
-      case 1: collocate_columns<1>(T, lay, wp, W, ncol, dst, npts, lane); break;
-      case 2: collocate_columns<2>(T, lay, wp, W, ncol, dst, npts, lane); break;
-      case 3: collocate_columns<3>(T, lay, wp, W, ncol, dst, npts, lane); break;
-      case 4: collocate_columns<4>(T, lay, wp, W, ncol, dst, npts, lane); break;
-      case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(T, lay, wp, W, ncol, dst, npts, lane); break;
-      case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(T, lay, wp, W, ncol, dst, npts, lane); break;
+      case 0: collocate_columns<0>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 1: collocate_columns<1>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 2: collocate_columns<2>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 3: collocate_columns<3>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 4: collocate_columns<4>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
       default: break;  // the host routes larger lp to the per-task kernels
     }
     wave_sync();
@@ -485,91 +466,113 @@ __global__ __launch_bounds__(512) void collocate_wave_kernel(const Task *__restr
   }
 }
 
-// Waves per SIMD the register allocation must allow: the force/virial
-// getters of FULL need more registers than the hab-only finish.
-template <int LPB, bool FULL>
-constexpr int integrate_occupancy() { return FULL ? (LPB <= 4 ? 2 : 1) : (LPB <= 4 ? 3 : 2); }
-
-template <int LPB, bool FULL>
-__global__ __launch_bounds__(256, (integrate_occupancy<LPB, FULL>())) void integrate_wave_kernel(
-    const Task *__restrict__ tasks, const Level *__restrict__ levels,
-    const Group *__restrict__ groups, PointParams P) {
+template <int LPB>
+__global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel(
+    const Task *__restrict__ tasks, const Level *__restrict__ levels, PointParams P) {
   extern __shared__ double lds[];
   const WavePlan &wp = P.wp;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
-  const long long t = (long long)blockIdx.x * nw + wave;
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
   const long long task = P.task_base + t;
   const Task &T = tasks[task];
+  const GeoClass &G = P.geo[T.geo];
   const Level &L = levels[wp.level];
   const Layout &lay = L.lay;
-  const int npts = (int)L.npts;
   const WaveLds W = wave_lds(lds, wp, wave);
-  const int la = T.la_max + P.ld[0], lb = T.lb_max + P.ld[2];
-  const int la_min = max(T.la_min + P.ld[1], 0), lb_min = max(T.lb_min + P.ld[3], 0);
-  const int lp = la + lb;
-  const int ncol = wave_geometry(T, lay, wp, W, lane);
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
   wave_tables(T, lay, lp, wp, W.u, lane);
+  double *out = P.cxyz_all + task * P.cxyz_stride;
   const double *grid = L.grid;
   switch (lp) {
-    case 0: integrate_moments<0>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    case 1: integrate_moments<1>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    case 2: integrate_moments<2>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    case 3: integrate_moments<3>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    case 4: integrate_moments<4>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    case 5: if (LPB >= 5) integrate_moments<(LPB >= 5 ? 5 : 0)>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    case 6: if (LPB >= 6) integrate_moments<(LPB >= 6 ? 6 : 0)>(T, lay, wp, W, ncol, grid, npts, lane); break;
-    default: return;  // the host routes larger lp to the per-task kernels
+    case 0: integrate_task<0>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 1: integrate_task<1>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 2: integrate_task<2>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 3: integrate_task<3>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 4: integrate_task<4>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    default: break;  // the host routes larger lp to the per-task kernels
   }
-  wave_finish<FULL>(P, T, groups[T.group], la, lb, la_min, lb_min, wp, W, task, lane);
+}
+
+// Packed cxyz of one task per wave -> hab, forces, virial.
+template <bool FULL>
+__global__ __launch_bounds__(256, (FULL ? 2 : 4)) void finish_wave_kernel(
+    const Task *__restrict__ tasks, const Group *__restrict__ groups, PointParams P) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= P.ntasks) return;
+  const long long task = P.task_base + t;
+  const Task &T = tasks[task];
+  const TaskL l = task_l(P, T);
+  const WaveLds W = wave_lds(lds, P.wp, wave);
+  const int NP = P.wp.np;
+  const double *in = P.cxyz_all + task * P.cxyz_stride;
+  for (int e = lane; e < NP * NP * NP; e += 64) {
+    const int lz = e / (NP * NP), ly = (e / NP) % NP, lx = e % NP;
+    W.cxyz[e] = (lx + ly + lz <= l.lp) ? in[packed_index(lx, ly, lz, l.lp)] : 0.0;
+  }
+  wave_sync();
+  wave_finish<FULL>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, P.wp, W, task, lane);
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <typename K>
-static hipError_t launch_wave(K kernel, const PointParams &P, int blocks, hipStream_t s) {
+template <typename K, typename... A>
+static hipError_t launch_wave(K kernel, const PointParams &P, int blocks, hipStream_t s,
+                              A... args) {
   const size_t lds = (size_t)P.wp.bytes;
   if (lds > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * P.wp.nwaves), lds, s, P.tasks, P.levels,
-                     P.groups, P);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * P.wp.nwaves), lds, s, args..., P);
   return hipGetLastError();
 }
 
-// Collocation (or point counting) of P.ntasks tasks from P.task_base; lpb is
-// 4 or 6; blocks is the grid size (any size: waves loop over the range).
+static int wave_blocks(const PointParams &P) { return (P.ntasks + P.wp.nwaves - 1) / P.wp.nwaves; }
+
+hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
+  return launch_wave(cxyz_wave_kernel, P, wave_blocks(P), s, P.tasks, P.groups);
+}
+
+// blocks: grid size of the collocation (waves loop over the range).
 hipError_t launch_collocate_wave(const PointParams &P, int lpb, int blocks, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
-  if (P.wp.nwaves < 1 || P.wp.nwaves > 8 || blocks < 1) return hipErrorInvalidValue;
+  if (P.wp.nwaves < 1 || blocks < 1) return hipErrorInvalidValue;
+  if (P.wp.nwaves > (P.wp.window ? (lpb <= 4 ? 12 : 8) : 4)) return hipErrorInvalidValue;
   const bool win = P.wp.window > 0;
   if (lpb == 4)
-    return win ? launch_wave(collocate_wave_kernel<4, true>, P, blocks, s)
-               : launch_wave(collocate_wave_kernel<4, false>, P, blocks, s);
+    return win ? launch_wave(collocate_wave_kernel<4, true>, P, blocks, s, P.tasks, P.levels)
+               : launch_wave(collocate_wave_kernel<4, false>, P, blocks, s, P.tasks, P.levels);
   if (lpb == 6)
-    return win ? launch_wave(collocate_wave_kernel<6, true>, P, blocks, s)
-               : launch_wave(collocate_wave_kernel<6, false>, P, blocks, s);
+    return win ? launch_wave(collocate_wave_kernel<6, true>, P, blocks, s, P.tasks, P.levels)
+               : launch_wave(collocate_wave_kernel<6, false>, P, blocks, s, P.tasks, P.levels);
   return hipErrorInvalidValue;
 }
 
-// Integration: one wave per task, ceil(ntasks / nwaves) workgroups.
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
-  const int blocks = (P.ntasks + P.wp.nwaves - 1) / P.wp.nwaves;
-  const bool full = P.do_forces || P.compute_tau;
-  if (lpb == 4)
-    return full ? launch_wave(integrate_wave_kernel<4, true>, P, blocks, s)
-                : launch_wave(integrate_wave_kernel<4, false>, P, blocks, s);
-  if (lpb == 6)
-    return full ? launch_wave(integrate_wave_kernel<6, true>, P, blocks, s)
-                : launch_wave(integrate_wave_kernel<6, false>, P, blocks, s);
+  if (lpb == 4) return launch_wave(integrate_wave_kernel<4>, P, wave_blocks(P), s, P.tasks, P.levels);
+  if (lpb == 6) return launch_wave(integrate_wave_kernel<6>, P, wave_blocks(P), s, P.tasks, P.levels);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_finish_wave(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
+  const bool full = P.do_forces || P.compute_tau;
+  return full ? launch_wave(finish_wave_kernel<true>, P, wave_blocks(P), s, P.tasks, P.groups)
+              : launch_wave(finish_wave_kernel<false>, P, wave_blocks(P), s, P.tasks, P.groups);
 }
 
 }  // namespace gm
