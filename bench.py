@@ -5,29 +5,55 @@ grid_integrate_task_list (potential -> Hamiltonian blocks) over the same task
 list, with inputs and outputs resident in HBM (the library's *_dev entry
 points).  value = task-points processed per second (collocate + integrate),
 where a task-point is one (task, grid point) pair inside the reference's
-truncation region, counted on the device with the kernels' own membership.
+truncation region.
 
 Default workload: 64 rigid H2O in a 12.4 A cube, 4 grid levels (128/72/42/24),
-DZVP-like contracted s/p/d shells, FP64 (the plain FP64 GPU collocate+
-integrate configuration).  Synthetic, seeded (cp2k_amd/synth.py).
+DZVP-like contracted s/p/d shells, FP64.  Synthetic, seeded (cp2k_amd/synth.py).
 
-Also reported: the roofline of the dominant kernel (HBM-bound, algorithmic
-bytes per launch / average launch time from HIP events recorded on the
-library's launch stream inside the timed region) and a CPU baseline: CP2K's
-own OpenMP cpu-backend kernels (oracle/_ref) on a bounded sample of the same
-workload, on this host's cores.
+Also reported:
+  roofline      of the dominant call (collocate or integrate), HBM-bound:
+                achieved = algorithmic bytes of one call / its average duration
+                from HIP events on the launch stream.  Algorithmic bytes:
+                collocate 16 B per task-point (FP64 read + write of the grid
+                value) + per task its Cartesian density sub-block and record +
+                8 B per level grid point (overwrite); integrate 8 B per
+                task-point + per task its hab sub-block and record.  The LDS
+                tiles make the measured bytes far smaller: traffic = HBM bytes
+                of one call from a committed rocprofv3 PMC summary
+                (profiles/), used only when it was measured on the current
+                kernel sources, else null.  Also reported: FP64 throughput of
+                the call (2 (lp + 1) flops per task-point) against the FP64
+                peak (78.6 TFLOP/s).
+  cpu_baseline  CP2K's own OpenMP cpu-backend kernels (oracle/_ref) on a bounded
+                sample of the same workload, on this host's cores.
+
+`python bench.py --rocprof-summary DIR` turns rocprofv3 output (kernel trace
+statistics and FETCH_SIZE / WRITE_SIZE counter runs of this script) into the
+text summaries kept under profiles/.
 """
 import argparse
+import csv
+import glob
+import hashlib
 import json
 import os
 import pathlib
+import re
 import sys
 import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
+
+# kernels of one collocate / integrate call, by rocprofv3 kernel name
+CALL_KERNELS = {
+    "collocate": ("cxyz_wave_kernel", "collocate_tile_kernel", "collocate_wave_kernel",
+                  "collocate_kernel"),
+    "integrate": ("integrate_wave_kernel", "finish_wave_kernel", "integrate_kernel"),
+}
 
 
 def parse():
@@ -44,15 +70,16 @@ def parse():
                    help="before the steps, run 3 device-to-device copies of MIB MiB: "
                         "a plain copy kernel of known size that scales the HBM "
                         "byte counters of a rocprofv3 --pmc run")
+    p.add_argument("--rocprof-summary", metavar="DIR",
+                   help="summarise rocprofv3 output under DIR and exit (no GPU)")
+    p.add_argument("--calls", type=int, default=1,
+                   help="with --rocprof-summary: collocate/integrate calls in the run "
+                        "(the copy calibration size is --copy-calibration, default 512)")
     return p.parse_args()
 
 
 def algorithmic_bytes(args, points, which):
-    """HBM bytes per launch of the point kernel:
-    collocate: 16 B per task-point (FP64 grid read + write) + per task its
-               Cartesian density sub-block and task record;
-    integrate: 8 B per task-point (grid read) + per task hab sub-block and
-               task record (+ density sub-block for forces)."""
+    """Algorithmic HBM bytes of one collocate / integrate call (see module doc)."""
     from cp2k_amd.tasklist import ncoset
     import numpy as np
     la = np.array([b.lmax[s - 1] for b, s in zip(
@@ -65,8 +92,81 @@ def algorithmic_bytes(args, points, which):
     sub = 8.0 * float(np.sum(nc(la) * nc(lb)))
     record = 280.0 * args.ntasks
     if which == "collocate":
-        return 16.0 * points + sub + record
+        grid_pts = float(sum(int(np.prod(n)) for n in args.npts_local))
+        return 16.0 * points + sub + record + 8.0 * grid_pts
     return 8.0 * points + sub + record
+
+
+def source_hash():
+    """Hash of the kernel sources a profile summary was measured on."""
+    h = hashlib.sha256()
+    for f in sorted((ROOT / "cp2k_amd" / "csrc").glob("*")):
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def call_of(name):
+    for call, keys in CALL_KERNELS.items():
+        if any(k + "<" in name or k + "(" in name for k in keys):
+            return call
+    return None
+
+
+def rocprof_summary(d, calls, copy_mib):
+    """Text summary of rocprofv3 output under d: kernel statistics, and HBM
+    traffic per collocate / integrate call from FETCH_SIZE and WRITE_SIZE
+    counters scaled by what they read for the copy calibration (a device copy
+    of known size, __amd_rocclr_copyBuffer)."""
+    out = [f"source_hash: {source_hash()}"]
+    for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)):
+        out.append(f"== kernel statistics ({os.path.relpath(f, d)})")
+        out.append(f"{'kernel':70s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'%':>6s}")
+        for r in csv.DictReader(open(f)):
+            out.append(f"{r['Name'][:70]:70s} {int(r['Calls']):6d} "
+                       f"{float(r['TotalDurationNs']) / 1e6:10.3f} "
+                       f"{float(r['AverageNs']) / 1e6:9.3f} {float(r['Percentage']):6.2f}")
+    per = {}  # counter -> {"copy": [...], call: total KiB}
+    for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        for r in csv.DictReader(open(f)):
+            c, v, n = r["Counter_Name"], float(r["Counter_Value"]), r["Kernel_Name"]
+            e = per.setdefault(c, {"copy": [], "collocate": 0.0, "integrate": 0.0})
+            if "copyBuffer" in n:
+                e["copy"].append(v)
+            elif call_of(n):
+                e[call_of(n)] += v
+    if "FETCH_SIZE" in per and "WRITE_SIZE" in per:
+        out.append("== HBM traffic per call (KiB from FETCH_SIZE / WRITE_SIZE, per dispatch summed)")
+        # the calibration copies (the largest copy dispatches) read and write
+        # copy_mib MiB each
+        copy_kib = 1024.0 * copy_mib
+        fetch_copy = max(per["FETCH_SIZE"]["copy"] or [0.0])
+        write_copy = max(per["WRITE_SIZE"]["copy"] or [0.0])
+        ff = copy_kib / fetch_copy if fetch_copy else float("nan")
+        wf = copy_kib / write_copy if write_copy else float("nan")
+        out.append(f"copy calibration: a {copy_kib:.0f} KiB device copy reads FETCH_SIZE "
+                   f"{fetch_copy:.0f} KiB and WRITE_SIZE {write_copy:.0f} KiB -> "
+                   f"fetch factor {ff:.3f}, write factor {wf:.3f}")
+        for call in ("collocate", "integrate"):
+            fk = per["FETCH_SIZE"][call] / calls
+            wk = per["WRITE_SIZE"][call] / calls
+            tb = (fk * ff + wk * wf) * 1024.0
+            out.append(f"call {call}: fetch {fk:.0f} KiB, write {wk:.0f} KiB, "
+                       f"traffic_bytes {tb:.0f}")
+    return "\n".join(out)
+
+
+def profile_traffic(call, workload):
+    """HBM bytes per call from a committed summary measured on these sources."""
+    f = ROOT / "profiles" / f"{workload}_hbm_traffic.txt"
+    if not f.exists():
+        return None
+    text = f.read_text()
+    m = re.search(r"source_hash: (\w+)", text)
+    if not m or m.group(1) != source_hash():
+        return None
+    m = re.search(rf"call {call}: .*traffic_bytes (\d+)", text)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(args, pab, V, seconds):
@@ -119,6 +219,9 @@ def cpu_baseline(args, pab, V, seconds):
 
 def main():
     opt = parse()
+    if opt.rocprof_summary:
+        print(rocprof_summary(opt.rocprof_summary, opt.calls, opt.copy_calibration or 512))
+        return
     if opt.gpus != 1:
         print(f"bench.py measures one GPU only (this backend has no multi-GPU path); "
               f"got --gpus {opt.gpus}", file=sys.stderr)
@@ -131,6 +234,7 @@ def main():
     gen_s = time.time() - t0
     gt = grid.GridTaskList(args)
     points = gt.task_points()
+    by_lp = gt.task_points_by_lp()
     pab_d = grid.DeviceArray.from_numpy(pab)
     rho_d = [grid.DeviceArray(s) for s in args.grid_shapes()]
     V_d = [grid.DeviceArray.from_numpy(v) for v in V]
@@ -164,15 +268,19 @@ def main():
     t1 = time.perf_counter()
     grid.set_timing(False)
     ms_step = (t1 - t0) * 1e3 / opt.steps
-    kt = {k: grid.kernel_time_ms(i) for i, k in enumerate(
-        ["collocate", "integrate", "decontract", "contract", "fill", "reduce"])}
+    kt = {k: grid.kernel_time_ms(i) for i, k in enumerate(grid.KERNEL_NAMES)}
     value = 2.0 * points / (ms_step * 1e-3) / 1e9
-    # dominant kernel
+    # dominant call and its FP64 roofline; lp shift of the call: collocate
+    # GRID_FUNC_AB 0, integrate hab 0 (forces 1, virial 2 on la/lb)
     dom = max(("collocate", "integrate"), key=lambda k: kt[k][0])
     ms, n = kt[dom]
     avg_ms = ms / max(n, 1)
+    shift = 0 if (dom == "collocate" or not opt.forces) else 2
+    flops = sum(2.0 * (lp + shift + 1) * p for lp, p in enumerate(by_lp))
+    tflops = flops / (avg_ms * 1e-3) / 1e12
     abytes = algorithmic_bytes(args, points, dom)
     achieved = abytes / (avg_ms * 1e-3) / 1e9
+    traffic = profile_traffic(dom, opt.config) if not opt.forces else None
     out = {
         "metric": "collocate+integrate Gpts/s & HBM GB/s; FP32-mixed vs FP64 at <=2e-6 rel rho err",
         "value": round(value, 3),
@@ -192,6 +300,7 @@ def main():
             "config_name": opt.config,
             "ntasks": args.ntasks,
             "task_points": points,
+            "task_points_by_lp": by_lp,
             "levels": [int(n[0]) for n in args.npts_local],
             "forces": bool(opt.forces),
             "tasks_per_s": round(2 * args.ntasks / (ms_step * 1e-3), 1),
@@ -199,14 +308,22 @@ def main():
         "kernel_ms_per_step": {k: round(v[0] / max(opt.steps, 1), 4) for k, v in kt.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": f"{dom} call (" + ", ".join(
+                k for k in grid.KERNEL_NAMES[6:] if kt[k][0] > 0 and
+                ((dom == "collocate") == (k in ("cxyz", "collocate_tiles", "collocate_direct",
+                                                 "per_task")) or k == "per_task")) + ")",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": abytes,
             "avg_launch_ms": round(avg_ms, 4),
+            "measured_hbm_gbs": (round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+                                 if traffic else None),
+            "fp64_tflops": round(tflops, 3),
+            "fp64_frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+            "algorithmic_flops_per_launch": flops,
         },
         "cpu_baseline": None,
     }

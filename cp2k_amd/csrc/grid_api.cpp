@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <array>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -41,7 +42,8 @@ hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *ta
 hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
                                 hipStream_t s);
 hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s);
-hipError_t launch_collocate_wave(const PointParams &P, int lpb, int blocks, hipStream_t s);
+hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s);
+hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
 }  // namespace gm
@@ -78,7 +80,7 @@ static inline int pmod(int a, int m) { return ((a % m) + m) % m; }
 // Library state (src/grid/common/grid_library.c)
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kNumKernelIds = 6;
+constexpr int kNumKernelIds = 12;
 constexpr int kCounterLp = 20;
 
 struct PendingTiming {
@@ -144,14 +146,14 @@ struct TimedLaunch {
   hipEvent_t a = nullptr, b = nullptr;
   hipStream_t s;
   TimedLaunch(int id_, hipStream_t s_) : id(id_), s(s_) {
-    if (S.timing) {
+    if (S.timing && id >= 0) {
       a = get_event();
       b = get_event();
       HIP_CHECK(hipEventRecord(a, s));
     }
   }
   ~TimedLaunch() {
-    if (S.timing) {
+    if (a) {
       (void)hipEventRecord(b, s);
       S.pending.push_back({id, a, b});
     }
@@ -249,7 +251,26 @@ struct OrthoLevel {
   bool classed = true;  // every task has a geometry class
   long long points = 0;  // task points of the level (classed levels)
   int lp_start[kLp0Slots] = {};
+  // collocation tiles (classed levels): nx * by * bz points
+  bool tiled = false;
+  int by = 0, bz = 0;
 };
+
+// Items of one tile, sorted by lp0: start[v] = first item with lp0 >= v.
+struct TileInfo {
+  int level, gy0, gz0;
+  int start[kLp0Slots + 1];
+};
+
+// Workgroups of the tile kernels for one lp shift: per level and lp bucket
+// (lp <= 4, lp 5..6) a range of TileUnit records in `d`.
+struct TilePlan {
+  DevBuf d;
+  std::vector<std::array<std::pair<int, int>, 2>> range;  // [level][bucket] = (first, count)
+};
+
+// Largest LDS tile in doubles; the rest of LDS holds the wave regions.
+constexpr long long kTileDoubles = 14336;
 
 // Orthorhombic row start, src/grid/ref/grid_ref_collint.h:147 / :49 (the
 // device copy is ortho_start in grid_device.h).
@@ -268,6 +289,9 @@ struct TaskListImpl {
   // device-resident description
   DevBuf d_tasks, d_groups, d_sphi, d_fptr, d_fent, d_levels, d_cart, d_hab_cart, d_task_out;
   DevBuf d_geo, d_geo_cols, d_geo_kmin, d_cxyz;  // geometry classes, per-task cxyz
+  DevBuf d_geo_rows, d_items;                      // row starts, collocation tile items
+  std::vector<TileInfo> tiles;
+  std::map<int, TilePlan> tile_plans;              // by lp shift
   Level *h_levels = nullptr;  // pinned staging for the per-call level table
   long long cart_len = 0;
   // maxima for the LDS plan
@@ -282,7 +306,9 @@ struct TaskListImpl {
   std::vector<signed char> lp0s;  // lp0 per sorted task
   int ngeo = 0;
   long long class_points = 0;     // task points of the tasks with a geometry class
+  std::vector<long long> class_npts;  // per sorted task: points of its class (or 0)
   long long points = -1;
+  std::vector<long long> points_by_lp;  // task points per la_max + lb_max
   // per-call scratch when the caller's buffers have no device mirror
   DevBuf pab_dev, hab_dev, forces_dev;
   std::vector<DevBuf> grid_dev;
@@ -290,7 +316,8 @@ struct TaskListImpl {
   ~TaskListImpl() {
     DevBuf *bufs[] = {&d_tasks, &d_groups, &d_sphi, &d_fptr, &d_fent, &d_levels,
                       &d_cart, &d_hab_cart, &d_task_out, &pab_dev, &hab_dev, &forces_dev,
-                      &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz};
+                      &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz, &d_geo_rows, &d_items};
+    for (auto &kv : tile_plans) kv.second.d.release();
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
     if (h_levels) (void)hipHostFree(h_levels);
@@ -615,7 +642,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   //     j-major, with the first plane kl of each column (planes kl .. 1 - kl),
   //     from the bounds of src/grid/ref/grid_ref_collint.h:139-167 ---
   std::vector<GeoClass> geo;
-  std::vector<int> geo_cols, geo_kmin;
+  std::vector<int> geo_cols, geo_kmin, geo_rows;
   std::map<std::pair<int, unsigned long long>, int> class_of;
   tl->class_points = 0;
   const int kNoRow = std::numeric_limits<int>::max();
@@ -633,6 +660,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
       GeoClass G{};
       G.col_off = (int)geo_cols.size();
       G.grp_off = (int)geo_kmin.size();
+      G.row_off = (int)geo_rows.size();
       for (int d = 0; d < 3; d++) G.lo[d] = T.lo[d];
       const int nkz = 1 - T.lo[2], njy = 1 - T.lo[1];
       std::vector<int> i0((size_t)nkz * njy);
@@ -652,6 +680,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
         }
       }
       for (int j = T.lo[1]; j <= 1 - T.lo[1]; j++) {
+        geo_rows.push_back((int)geo_cols.size() - G.col_off);
         const int jy = (j <= 0 ? j : 1 - j) - T.lo[1];
         const int ic = i0[(size_t)(nkz - 1) * njy + jy];
         if (ic == kNoRow) continue;
@@ -667,6 +696,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
         }
       }
       G.ncol = (int)geo_cols.size() - G.col_off;
+      geo_rows.push_back(G.ncol);
       for (int c0 = 0; c0 < G.ncol; c0 += 64) {
         int kmin = 1;
         for (int c = c0; c < std::min(c0 + 64, G.ncol); c++)
@@ -679,6 +709,9 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     T.geo = it->second;
   }
   tl->ngeo = (int)geo.size();
+  tl->class_npts.assign(ntasks, 0);
+  for (int s = 0; s < ntasks; s++)
+    if (tasks[s].geo >= 0) tl->class_npts[s] = geo[tasks[s].geo].npts;
 
   // --- per-level ranges of the orthorhombic tasks ---
   tl->olev.assign(nlevels, OrthoLevel{});
@@ -698,6 +731,92 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   }
   for (auto &o : tl->olev)
     if (o.end > o.begin && o.classed) tl->class_points += o.points;
+
+  // --- collocation tiles: per classed level, tiles of full rows; each task
+  //     becomes one item per tile its cube reaches (runs of rows j and planes
+  //     k whose grid index stays in one tile without wrapping) ---
+  std::vector<TileItem> items;
+  tl->tiles.clear();
+  for (auto &kv : tl->tile_plans) kv.second.d.release();
+  tl->tile_plans.clear();
+  for (int l = 0; l < nlevels; l++) {
+    OrthoLevel &o = tl->olev[l];
+    o.tiled = false;
+    if (o.end <= o.begin || !o.classed) continue;
+    const Layout &y = tl->layouts[l];
+    const long long nx = y.npts_local[0], ny = y.npts_local[1], nz = y.npts_local[2];
+    if (nx * ny * nz <= kTileDoubles) {
+      o.by = (int)ny;
+      o.bz = (int)nz;
+    } else if (kTileDoubles / (nx * nz) >= 4) {
+      o.by = (int)(kTileDoubles / (nx * nz));
+      o.bz = (int)nz;
+    } else if (nx * 8 * 2 <= kTileDoubles) {
+      o.by = (int)std::min<long long>(8, ny);
+      o.bz = (int)std::min<long long>(nz, kTileDoubles / (nx * o.by));
+    } else {
+      continue;  // rows too long for a tile: straight into HBM
+    }
+    o.tiled = true;
+    const bool full_y = o.by == ny, full_z = o.bz == nz;
+    const int nty = (int)((ny + o.by - 1) / o.by);
+    struct Seg { int a, b, t; };
+    auto segments = [&](int lo, int cw, int nglob, int nloc, int bsz, bool full, std::vector<Seg> &out) {
+      out.clear();
+      int prev = -2;
+      for (int c = lo; c <= 1 - lo; c++) {
+        const int g = pmod(cw + c, nglob);
+        if (g >= nloc) fatal("level %d: cube leaves the local grid", l + 1);
+        const int t = full ? 0 : g / bsz;
+        if (out.empty() || t != out.back().t || (!full && g != prev + 1))
+          out.push_back(Seg{c, c, t});
+        else
+          out.back().b = c;
+        prev = g;
+      }
+    };
+    struct Keyed { long long key; TileItem it; };
+    std::vector<Keyed> lev_items;
+    std::vector<Seg> ys, zs;
+    for (int s2 = o.begin; s2 < o.end; s2++) {
+      const Task &T = tasks[s2];
+      segments(T.lo[1], pmod(T.center[1] - y.shift_local[1], y.npts_global[1]),
+               y.npts_global[1], (int)ny, o.by, full_y, ys);
+      segments(T.lo[2], pmod(T.center[2] - y.shift_local[2], y.npts_global[2]),
+               y.npts_global[2], (int)nz, o.bz, full_z, zs);
+      for (const Seg &a : ys)
+        for (const Seg &b : zs) {
+          TileItem it{};
+          it.task = s2;
+          it.j0 = (short)a.a;
+          it.j1 = (short)a.b;
+          it.k0 = (short)b.a;
+          it.k1 = (short)b.b;
+          const long long tile = (long long)b.t * nty + a.t;
+          lev_items.push_back(Keyed{tile << 8 | tl->lp0s[s2], it});
+        }
+    }
+    std::stable_sort(lev_items.begin(), lev_items.end(),
+                     [](const Keyed &a, const Keyed &b) { return a.key < b.key; });
+    for (size_t a = 0; a < lev_items.size();) {
+      const long long tile = lev_items[a].key >> 8;
+      size_t b = a;
+      while (b < lev_items.size() && (lev_items[b].key >> 8) == tile) b++;
+      TileInfo ti{};
+      ti.level = l;
+      ti.gy0 = (int)(tile % nty) * o.by;
+      ti.gz0 = (int)(tile / nty) * o.bz;
+      const int first = (int)items.size();
+      for (size_t q = a; q < b; q++) items.push_back(lev_items[q].it);
+      for (int v = 0; v <= kLp0Slots; v++) {
+        int q = first;
+        while (q < (int)items.size() && tl->lp0s[items[q].task] < v) q++;
+        ti.start[v] = q;
+      }
+      tl->tiles.push_back(ti);
+      a = b;
+    }
+  }
   for (auto &o : tl->olev)
     for (int v = 0; v < kLp0Slots; v++) {
       int s = o.begin;
@@ -726,6 +845,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   upload(tl->d_geo, geo);
   upload(tl->d_geo_cols, geo_cols);
   upload(tl->d_geo_kmin, geo_kmin);
+  upload(tl->d_geo_rows, geo_rows);
+  upload(tl->d_items, items);
   tl->d_levels.get(sizeof(Level) * nlevels);
   if (!tl->h_levels) HIP_CHECK(hipHostMalloc((void **)&tl->h_levels, sizeof(Level) * 64));
   if (nlevels > 64) fatal("at most 64 grid levels are supported");
@@ -846,15 +967,17 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
   w.wave_bytes = (8 * nd + 15) & ~15;
   const int limit = lds_limit();
   if (kind == kPlanWindow) {
-    const long long npts = (long long)tl->npts_local[3 * level] * tl->npts_local[3 * level + 1] *
-                           tl->npts_local[3 * level + 2];
-    const int most = lp <= 4 ? 12 : 8;  // launch bounds of the window kernels
+    const long long npts = (long long)tl->npts_local[3 * level] * o.by * o.bz;
+    const int most = lp <= 4 ? 12 : 8;  // launch bounds of the tile kernels
     for (int nw : {12, 10, 8, 6, 4})
       if (nw <= most && 8 * npts + (long long)nw * w.wave_bytes <= limit) {
         w.window = (int)npts;
+        w.by = o.by;
+        w.bz = o.bz;
         w.nwaves = nw;
         break;
       }
+    if (!w.window) fatal("level %d: no room for the collocation tile in LDS", level + 1);
   }
   if (!w.window) {
     w.nwaves = 1;
@@ -877,12 +1000,48 @@ int cxyz_stride(const TaskListImpl *tl, int sh) {
   return (lp + 1) * (lp + 2) * (lp + 3) / 6;
 }
 
+// Workgroups of the tile kernels for lp shift sh: the items of each tile in
+// an lp bucket are dealt to a number of workgroups proportional to their
+// count, about one workgroup per CU over a level.
+TilePlan &tile_plan(TaskListImpl *tl, int sh) {
+  auto it = tl->tile_plans.find(sh);
+  if (it != tl->tile_plans.end()) return it->second;
+  TilePlan &plan = tl->tile_plans[sh];
+  plan.range.assign(tl->nlevels, {std::make_pair(0, 0), std::make_pair(0, 0)});
+  std::vector<TileUnit> units;
+  auto slot = [](int v) { return std::min(std::max(v, 0), kLp0Slots); };
+  for (int l = 0; l < tl->nlevels; l++) {
+    for (int r = 0; r < 2; r++) {
+      const int v0 = slot(r == 0 ? 0 : 5 - sh), v1 = slot(r == 0 ? 5 - sh : 7 - sh);
+      long long total = 0;
+      for (const TileInfo &t : tl->tiles)
+        if (t.level == l) total += t.start[v1] - t.start[v0];
+      const int first = (int)units.size();
+      if (total > 0) {
+        const long long per = std::max<long long>(32, (total + num_cus() - 1) / num_cus());
+        for (const TileInfo &t : tl->tiles) {
+          if (t.level != l) continue;
+          const int a = t.start[v0], b = t.start[v1];
+          if (b <= a) continue;
+          const int nw = (int)std::max<long long>(1, ((long long)(b - a) + per / 2) / per);
+          for (int w = 0; w < nw; w++) units.push_back(TileUnit{a, b, w, nw, t.gy0, t.gz0});
+        }
+      }
+      plan.range[l][r] = std::make_pair(first, (int)units.size() - first);
+    }
+  }
+  upload(plan.d, units);
+  return plan;
+}
+
 // Orthorhombic tasks, level by level: wave-per-task kernels for the levels
 // whose tasks all have a geometry class, for lp <= 6; the per-task kernels
 // for the rest.  Collocate: cab -> cxyz pass, then the point kernels;
 // integrate: the point kernels, then the cxyz -> hab/forces pass.
-void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   const int sh = P.ld[0] + P.ld[2];
+  const TilePlan *plan = mode == kCollocate ? &tile_plan(tl, sh) : nullptr;
+  if (plan) P.units = (const TileUnit *)plan->d.p;
   for (int l = 0; l < tl->nlevels; l++) {
     const OrthoLevel &o = tl->olev[l];
     if (o.end <= o.begin) continue;
@@ -895,7 +1054,10 @@ void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode
       Q.task_base = o.begin;
       Q.ntasks = c2 - o.begin;
       Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanSetup);
-      if (mode == kCollocate) HIP_CHECK(launch_cxyz_wave(Q, s));
+      if (mode == kCollocate) {
+        TimedLaunch tm(6, s);
+        HIP_CHECK(launch_cxyz_wave(Q, s));
+      }
       const int cuts[2][2] = {{o.begin, c1}, {c1, c2}};
       for (int r = 0; r < 2; r++) {
         const int a = cuts[r][0], b = cuts[r][1];
@@ -907,23 +1069,29 @@ void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode
         R.ntasks = b - a;
         if (mode == kIntegrate) {
           R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanPoints);
+          TimedLaunch tm(9, s);
           HIP_CHECK(launch_integrate_wave(R, lpb, s));
-        } else {
+        } else if (o.tiled) {
           R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanWindow);
-          int blocks = (R.ntasks + R.wp.nwaves - 1) / R.wp.nwaves;
-          if (R.wp.window) {
-            const int per_cu = std::max(1, lds_limit() / R.wp.bytes);
-            blocks = std::min(blocks, num_cus() * per_cu);
-          }
-          HIP_CHECK(launch_collocate_wave(R, lpb, blocks, s));
+          R.unit_base = plan->range[l][r].first;
+          TimedLaunch tm(7, s);
+          HIP_CHECK(launch_collocate_tiles(R, lpb, plan->range[l][r].second, s));
+        } else {
+          R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanPoints);
+          TimedLaunch tm(8, s);
+          HIP_CHECK(launch_collocate_wave(R, lpb, s));
         }
       }
-      if (mode == kIntegrate) HIP_CHECK(launch_finish_wave(Q, s));
+      if (mode == kIntegrate) {
+        TimedLaunch tm(10, s);
+        HIP_CHECK(launch_finish_wave(Q, s));
+      }
     }
     if (o.end > c2) {
       PointParams Q = P;
       Q.task_base = c2;
       Q.ntasks = o.end - c2;
+      TimedLaunch tm(11, s);
       HIP_CHECK(mode == kIntegrate ? launch_integrate(Q, true, s) : launch_collocate(Q, true, s));
     }
   }
@@ -931,7 +1099,7 @@ void launch_ortho(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode
 
 // Launches the orthorhombic and the general task ranges (tasks are sorted
 // by class: orthorhombic, general, skipped).
-void launch_ranges(const TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+void launch_ranges(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   if (mode == kCount) {
     // tasks with a geometry class are counted on the host
     for (int l = 0; l < tl->nlevels; l++) {
@@ -948,6 +1116,7 @@ void launch_ranges(const TaskListImpl *tl, PointParams P, hipStream_t s, int mod
   if (tl->n_general > 0) {
     P.task_base = (int)tl->n_ortho;
     P.ntasks = (int)tl->n_general;
+    TimedLaunch tm(mode == kCount ? -1 : 11, s);
     HIP_CHECK(mode == kIntegrate ? launch_integrate(P, false, s) : launch_collocate(P, false, s));
   }
 }
@@ -957,6 +1126,8 @@ void set_wave_buffers(TaskListImpl *tl, PointParams &P) {
   P.geo = (const GeoClass *)tl->d_geo.p;
   P.geo_cols = (const int *)tl->d_geo_cols.p;
   P.geo_kmin = (const int *)tl->d_geo_kmin.p;
+  P.geo_rows = (const int *)tl->d_geo_rows.p;
+  P.items = (const TileItem *)tl->d_items.p;
   P.cxyz_stride = cxyz_stride(tl, P.ld[0] + P.ld[2]);
   P.cxyz_all = (double *)tl->d_cxyz.get(sizeof(double) * (size_t)P.cxyz_stride *
                                         (size_t)std::max(tl->ntasks, 1));
@@ -1568,14 +1739,32 @@ long long grid_mi355x_task_points(const grid_task_list *task_list) {
   sync_stream(s);
   // classed levels from the geometry classes, the rest from the kernel counts
   long long total = tl->class_points;
+  tl->points_by_lp.assign(kLp0Slots, 0);
+  for (const OrthoLevel &o : tl->olev)
+    if (o.end > o.begin && o.classed)
+      for (int t = o.begin; t < o.end; t++)
+        tl->points_by_lp[tl->lp0s[t]] += tl->class_npts[t];
   auto add = [&](long long a, long long b) {
-    for (long long t = a; t < b; t++) total += (long long)out[(size_t)t * kTaskOutStride];
+    for (long long t = a; t < b; t++) {
+      const long long n = (long long)out[(size_t)t * kTaskOutStride];
+      total += n;
+      tl->points_by_lp[tl->lp0s[t]] += n;
+    }
   };
   for (const OrthoLevel &o : tl->olev)
     if (o.end > o.begin && !o.classed) add(o.begin, o.end);
   add(tl->n_ortho, tl->n_ortho + tl->n_general);
   tl->points = total;
   return total;
+}
+
+void grid_mi355x_task_points_by_lp(const grid_task_list *task_list, long long *points,
+                                   const int n) {
+  TaskListImpl *tl = checked(task_list);
+  for (int v = 0; v < n; v++) points[v] = 0;
+  if (tl->empty) return;
+  grid_mi355x_task_points(task_list);
+  for (int v = 0; v < n && v < (int)tl->points_by_lp.size(); v++) points[v] = tl->points_by_lp[v];
 }
 
 void grid_mi355x_task_counts(const grid_task_list *task_list, long *ortho, long *general,

@@ -60,8 +60,24 @@ struct Task {
 struct GeoClass {
   int col_off, ncol;   // into the column array
   int grp_off;         // into the group minimum array
+  int row_off;         // into the row start array: first column of row j, j = lo_y .. 2 - lo_y
   int lo[3];
   long long npts;      // points of one task of the class
+};
+
+// Collocation tiles: a level grid is cut into tiles of nx * by * bz points
+// (full rows along x) that fit into LDS.  An item is the part of one task
+// whose rows j0..j1 and planes k0..k1 (cube indices) map into one tile.  A
+// workgroup takes the items ibeg + first + m * stride < iend (m = 0, 1, ...)
+// of one tile, so the items of a tile are dealt evenly to its workgroups.
+struct TileItem {
+  int task;
+  short j0, j1, k0, k1;
+  int pad_;
+};
+struct TileUnit {
+  int ibeg, iend, first, stride;
+  int gy0, gz0;        // tile origin in the local grid
 };
 
 // One (matrix block, iset, jset) sub-block: the unit of (de)contraction.
@@ -109,7 +125,8 @@ struct WavePlan {
   int ext;         // per-axis table stride (max cube extent 2 - 2 lo)
   int nwaves;      // waves per workgroup
   int wave_bytes;  // LDS bytes per wave
-  int window;      // doubles of the LDS level window (0: accumulate in HBM)
+  int window;      // doubles of the LDS tile window (0: accumulate in HBM)
+  int by, bz;      // tile rows and planes (window = nx * by * bz)
   int o_cxyz, o_u; // doubles: cxyz [np^3]; {cab, alpha} (setup/finish) or tables [3][ext][np]
   int bytes;       // dynamic LDS per workgroup
 };
@@ -136,6 +153,10 @@ struct PointParams {
   const int *geo_kmin;
   double *cxyz_all;       // per task: cxyz packed (lz, ly, lx), lx fastest, lx+ly+lz <= lp
   int cxyz_stride;        // doubles per task
+  const int *geo_rows;
+  const TileUnit *units;  // collocation tiles
+  const TileItem *items;
+  int unit_base;          // first unit of the launch (one unit per workgroup)
 };
 
 }  // namespace gm

@@ -21,9 +21,11 @@
 // from the wave's LDS region.
 //
 // Collocate: per column D[lz] = sum_{lx,ly} cxyz px py, then per point
-// (lp + 1) fused multiply-adds and one atomic add, either into an LDS copy of
-// the whole level grid shared by the workgroup (added to HBM once at the end)
-// or straight into HBM.  Integrate: per point (lp + 1) multiply-adds of the z
+// (lp + 1) fused multiply-adds and one atomic add, into an LDS tile of the
+// level grid (full rows along x) that a workgroup fills from many tasks and
+// adds to HBM once; each task contributes the rows and planes of its cube
+// that fall into the tile (an item).  Levels with rows too long for any tile
+// add straight into HBM.  Integrate: per point (lp + 1) multiply-adds of the z
 // moments, per column their outer product with px, py into lane-private cxyz
 // accumulators, one cross-lane reduction per task.
 //
@@ -61,6 +63,12 @@ __device__ __forceinline__ double wave_total(double v) {
   v += dpp_mov<0x140, 0xf>(v);  // row_mirror
   v += dpp_mov<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
   v += dpp_mov<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -250,6 +258,67 @@ __device__ __forceinline__ void collocate_columns(const PointParams &P, const Ta
   }
 }
 
+// Collocate the rows j0..j1 and planes k0..k1 of a task into an LDS tile
+// win[(z - gz0) * nx * by + (y - gy0) * nx + x].
+template <int LP>
+__device__ __forceinline__ void tile_columns(const PointParams &P, const Task &T,
+                                             const GeoClass &G, const Layout &lay,
+                                             const WavePlan &wp, const double *tab,
+                                             const double *cx, const TileItem &I, int gy0,
+                                             int gz0, double *win, int lane) {
+  const int NP = wp.np;
+  const double *tabz = tab + 2 * wp.ext * NP;
+  const int nx = lay.npts_local[0];
+  const int nxby = nx * wp.by;
+  const int ngz = lay.npts_global[2];
+  int cw[3];
+  axis_offsets(T, lay, cw);
+  const int c_beg = P.geo_rows[G.row_off + (I.j0 - G.lo[1])];
+  const int c_end = P.geo_rows[G.row_off + (I.j1 + 1 - G.lo[1])];
+  for (int c0 = c_beg; c0 < c_end; c0 += 64) {
+    const int c = c0 + lane;
+    Column C = load_column(P, G, lay, cw, c < c_end ? c : G.ncol);
+    int lbase = 0;
+    if (C.kl <= 0) {
+      const int ly = C.base / nx - gy0;  // base = y * nx + x
+      if ((unsigned)ly < (unsigned)wp.by)
+        lbase = C.base - gy0 * nx;
+      else
+        C.kl = 1;  // outside the tile: never for items built on the host
+    }
+    double px[LP + 1], py[LP + 1], D[LP + 1];
+    load_pxy<LP>(T, wp, tab, C, px, py);
+    int e = 0;
+#pragma unroll
+    for (int lz = 0; lz <= LP; lz++) {
+      double s = 0.0;
+#pragma unroll
+      for (int ly = 0; ly <= LP - lz; ly++) {
+        double t = 0.0;
+#pragma unroll
+        for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fma(cx[e], px[lx], t);
+        s = fma(t, py[ly], s);
+      }
+      D[lz] = s;
+    }
+    const int kmin = __builtin_amdgcn_readfirstlane(wave_min(C.kl));
+    const int ka = max(kmin, (int)I.k0), kb = min(1 - kmin, (int)I.k1);
+    const unsigned span = (unsigned)(2 - 2 * C.kl);  // 0 for an idle lane
+    int gz = wrap(cw[2] + ka, ngz);
+#pragma unroll 2
+    for (int k = ka; k <= kb; k++) {
+      const double *pz = tabz + (k - T.lo[2]) * NP;
+      double v = 0.0;
+#pragma unroll
+      for (int lz = 0; lz <= LP; lz++) v = fma(D[lz], pz[lz], v);
+      const int tz = gz - gz0;
+      if ((unsigned)tz < (unsigned)wp.bz && (unsigned)(k - C.kl) < span)
+        atomicAdd(&win[tz * nxby + lbase], v);
+      gz = (gz + 1 == ngz) ? 0 : gz + 1;
+    }
+  }
+}
+
 // Integrate the task's points into lane-private accumulators acc[(lz, ly, lx)].
 template <int LP>
 __device__ __forceinline__ void integrate_columns(const PointParams &P, const Task &T, const GeoClass &G,
@@ -417,52 +486,90 @@ __global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__
   }
 }
 
-template <int LPB, bool WINDOW>
-__global__ __launch_bounds__(WINDOW ? (LPB <= 4 ? 768 : 512) : 256) void collocate_wave_kernel(const Task *__restrict__ tasks,
+// Collocation straight into HBM, one task per wave.
+template <int LPB>
+__global__ __launch_bounds__(256) void collocate_wave_kernel(const Task *__restrict__ tasks,
                                                              const Level *__restrict__ levels,
                                                              PointParams P) {
   extern __shared__ double lds[];
   const WavePlan &wp = P.wp;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nw = blockDim.x >> 6;
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= P.ntasks) return;
+  const long long task = P.task_base + t;
+  const Task &T = tasks[task];
+  const GeoClass &G = P.geo[T.geo];
   const Level &L = levels[wp.level];
   const Layout &lay = L.lay;
-  if (WINDOW) {
-    for (int e = threadIdx.x; e < wp.window; e += blockDim.x) lds[e] = 0.0;
-    __syncthreads();
-  }
   const WaveLds W = wave_lds(lds, wp, wave);
-  double *dst = WINDOW ? lds : L.grid;
-  const long long stride = (long long)gridDim.x * nw;
-  for (long long t = (long long)blockIdx.x * nw + wave; t < P.ntasks; t += stride) {
-    const long long task = P.task_base + t;
-    const Task &T = tasks[task];
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  const double *src = P.cxyz_all + task * P.cxyz_stride;
+  for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
+  wave_tables(T, lay, lp, wp, W.u, lane);  // ends with wave_sync
+  const double *cx = W.cxyz;
+  double *dst = L.grid;
+  switch (lp) {
+    case 0: collocate_columns<0>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 1: collocate_columns<1>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 2: collocate_columns<2>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 3: collocate_columns<3>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 4: collocate_columns<4>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    default: break;  // the host routes larger lp to the per-task kernels
+  }
+}
+
+// Collocation through LDS tiles: workgroup = unit P.unit_base + blockIdx.x,
+// the items U.ibeg + U.first + m * U.stride < U.iend of one tile.
+template <int LPB>
+__global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
+    const Task *__restrict__ tasks, const Level *__restrict__ levels,
+    const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
+  extern __shared__ double lds[];
+  const WavePlan &wp = P.wp;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  const TileUnit U = units[P.unit_base + blockIdx.x];
+  const Level &L = levels[wp.level];
+  const Layout &lay = L.lay;
+  for (int e = threadIdx.x; e < wp.window; e += blockDim.x) lds[e] = 0.0;
+  __syncthreads();
+  const WaveLds W = wave_lds(lds, wp, wave);
+  for (long long it = U.ibeg + U.first + (long long)wave * U.stride; it < U.iend;
+       it += (long long)nw * U.stride) {
+    const TileItem I = items[it];
+    const Task &T = tasks[I.task];
     const GeoClass &G = P.geo[T.geo];
     const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
     const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
-    const double *src = P.cxyz_all + task * P.cxyz_stride;
+    const double *src = P.cxyz_all + (long long)I.task * P.cxyz_stride;
     for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
     wave_tables(T, lay, lp, wp, W.u, lane);  // ends with wave_sync
     const double *cx = W.cxyz;
     switch (lp) {
-      case 0: collocate_columns<0>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-      case 1: collocate_columns<1>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-      case 2: collocate_columns<2>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-      case 3: collocate_columns<3>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-      case 4: collocate_columns<4>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-      case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-      case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+      case 0: tile_columns<0>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 1: tile_columns<1>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 2: tile_columns<2>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 3: tile_columns<3>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 4: tile_columns<4>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 5: if (LPB >= 5) tile_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 6: if (LPB >= 6) tile_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
       default: break;  // the host routes larger lp to the per-task kernels
     }
     wave_sync();
   }
-  if (WINDOW) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < wp.window; e += blockDim.x) {
-      const double v = lds[e];
-      if (v != 0.0) atomicAdd(&L.grid[e], v);
-    }
+  __syncthreads();
+  const int nx = lay.npts_local[0], ny = lay.npts_local[1], nz = lay.npts_local[2];
+  for (int e = threadIdx.x; e < wp.window; e += blockDim.x) {
+    const double v = lds[e];
+    if (v == 0.0) continue;
+    const int x = e % nx, r = e / nx;
+    const int y = U.gy0 + r % wp.by, z = U.gz0 + r / wp.by;
+    if (y < ny && z < nz) atomicAdd(&L.grid[((long long)z * ny + y) * nx + x], v);
   }
 }
 
@@ -544,18 +651,23 @@ hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s) {
   return launch_wave(cxyz_wave_kernel, P, wave_blocks(P), s, P.tasks, P.groups);
 }
 
-// blocks: grid size of the collocation (waves loop over the range).
-hipError_t launch_collocate_wave(const PointParams &P, int lpb, int blocks, hipStream_t s) {
+hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
-  if (P.wp.nwaves < 1 || blocks < 1) return hipErrorInvalidValue;
-  if (P.wp.nwaves > (P.wp.window ? (lpb <= 4 ? 12 : 8) : 4)) return hipErrorInvalidValue;
-  const bool win = P.wp.window > 0;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
+  if (lpb == 4) return launch_wave(collocate_wave_kernel<4>, P, wave_blocks(P), s, P.tasks, P.levels);
+  if (lpb == 6) return launch_wave(collocate_wave_kernel<6>, P, wave_blocks(P), s, P.tasks, P.levels);
+  return hipErrorInvalidValue;
+}
+
+// nunits workgroups from unit P.unit_base.
+hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s) {
+  if (nunits <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0)
+    return hipErrorInvalidValue;
   if (lpb == 4)
-    return win ? launch_wave(collocate_wave_kernel<4, true>, P, blocks, s, P.tasks, P.levels)
-               : launch_wave(collocate_wave_kernel<4, false>, P, blocks, s, P.tasks, P.levels);
+    return launch_wave(collocate_tile_kernel<4>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
   if (lpb == 6)
-    return win ? launch_wave(collocate_wave_kernel<6, true>, P, blocks, s, P.tasks, P.levels)
-               : launch_wave(collocate_wave_kernel<6, false>, P, blocks, s, P.tasks, P.levels);
+    return launch_wave(collocate_tile_kernel<6>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
   return hipErrorInvalidValue;
 }
 

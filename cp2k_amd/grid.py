@@ -33,7 +33,7 @@ EXPORTS = [
     "grid_mi355x_get_stream", "grid_mi355x_synchronize",
     "grid_mi355x_set_timing", "grid_mi355x_reset_timing",
     "grid_mi355x_kernel_time_ms", "grid_mi355x_task_points",
-    "grid_mi355x_task_counts",
+    "grid_mi355x_task_points_by_lp", "grid_mi355x_task_counts",
 ]
 
 GRID_FUNC_AB = 100
@@ -42,7 +42,11 @@ FP64 = 0
 FP32_MIXED = 1
 
 KERNEL_COLLOCATE, KERNEL_INTEGRATE, KERNEL_DECONTRACT, KERNEL_CONTRACT, \
-    KERNEL_FILL, KERNEL_REDUCE = range(6)
+    KERNEL_FILL, KERNEL_REDUCE, KERNEL_CXYZ, KERNEL_TILES, KERNEL_DIRECT, \
+    KERNEL_INTEGRATE_POINTS, KERNEL_FINISH, KERNEL_PER_TASK = range(12)
+KERNEL_NAMES = ["collocate", "integrate", "decontract", "contract", "fill", "reduce",
+                "cxyz", "collocate_tiles", "collocate_direct", "integrate_points",
+                "finish", "per_task"]
 
 
 class GridBasisSet(C.Structure):
@@ -88,6 +92,8 @@ def lib():
         L.grid_mi355x_kernel_time_ms.argtypes = [C.c_int, C.POINTER(C.c_long)]
         L.grid_mi355x_task_points.restype = C.c_longlong
         L.grid_mi355x_task_points.argtypes = [C.c_void_p]
+        L.grid_mi355x_task_points_by_lp.restype = None
+        L.grid_mi355x_task_points_by_lp.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]
         L.grid_mi355x_get_stream.restype = C.c_void_p
         L.grid_mi355x_set_stream.argtypes = [C.c_void_p]
         L.grid_mi355x_set_timing.argtypes = [C.c_bool]
@@ -291,6 +297,12 @@ class GridTaskList:
 
     def task_points(self):
         return int(lib().grid_mi355x_task_points(self.h))
+
+    def task_points_by_lp(self, n=11):
+        """Task points per la_max + lb_max of the tasks (index = lp)."""
+        out = (C.c_longlong * n)()
+        lib().grid_mi355x_task_points_by_lp(self.h, out, C.c_int(n))
+        return [int(x) for x in out]
 
     def task_counts(self):
         o, g, s = C.c_long(), C.c_long(), C.c_long()

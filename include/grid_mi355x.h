@@ -94,8 +94,11 @@ void *grid_mi355x_get_stream(void);
 void grid_mi355x_synchronize(void);
 
 /* Per-kernel timing with HIP events on the launch stream.  Kernel ids:
- * 0 collocate points, 1 integrate points, 2 decontract pab, 3 contract hab,
- * 4 zero/fill, 5 force/virial reduction. */
+ * 0 collocate (whole call: ids 6, 7, 8 and 11 of it), 1 integrate (whole
+ * call: ids 9, 10, 11 of it), 2 decontract pab, 3 contract hab, 4 zero/fill,
+ * 5 force/virial reduction, 6 cab -> cxyz pass, 7 collocation through LDS
+ * tiles, 8 collocation straight into HBM, 9 integrate points, 10 cxyz ->
+ * hab/forces pass, 11 per-task kernels (general cells, lp > 6). */
 void grid_mi355x_set_timing(const bool enable);
 void grid_mi355x_reset_timing(void);
 /* Returns total milliseconds and number of launches of kernel id. */
@@ -104,6 +107,10 @@ double grid_mi355x_kernel_time_ms(const int kernel_id, long *launches);
 /* Number of (task, grid point) pairs the list visits, counted on the device
  * with the same membership rules as collocate/integrate. */
 long long grid_mi355x_task_points(const grid_task_list *task_list);
+
+/* Task points split by la_max + lb_max of the task (points[v] for v < n). */
+void grid_mi355x_task_points_by_lp(const grid_task_list *task_list, long long *points,
+                                   const int n);
 
 /* Number of tasks that were assigned to the orthorhombic / general kernels. */
 void grid_mi355x_task_counts(const grid_task_list *task_list, long *ortho,
