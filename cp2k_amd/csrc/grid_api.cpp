@@ -42,6 +42,7 @@ hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *ta
 hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
                                 hipStream_t s);
 hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s);
+hipError_t launch_cxyz_lane(const PointParams &P, hipStream_t s);
 hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
@@ -304,6 +305,7 @@ struct TaskListImpl {
   long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
   std::vector<OrthoLevel> olev;   // per level
   std::vector<signed char> lp0s;  // lp0 per sorted task
+  std::vector<int> highl_before;  // sorted tasks before s with la_max or lb_max > 2
   int ngeo = 0;
   long long class_points = 0;     // task points of the tasks with a geometry class
   std::vector<long long> class_npts;  // per sorted task: points of its class (or 0)
@@ -421,16 +423,18 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   }
 
   // --- order: kernel class (orthorhombic, general, too small to map), then
-  //     level, la_max + lb_max (one polynomial degree bound per launch), block,
-  //     iset, jset (src/grid/ref/grid_ref_task_list.c:26-37) ---
+  //     level, la_max + lb_max (one polynomial degree bound per launch),
+  //     la_max (uniform waves in the lane-per-task setup), block, iset, jset
+  //     (src/grid/ref/grid_ref_task_list.c:26-37) ---
   std::vector<double> level_hmax(nlevels, 0.0);
   for (int l = 0; l < nlevels; l++)
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++)
         level_hmax[l] = fmax(level_hmax[l], fabs(tl->layouts[l].dh[i][j]));
-  std::vector<char> cls(ntasks), lpk(ntasks);
+  std::vector<char> cls(ntasks), lpk(ntasks), lak(ntasks);
   for (int t = 0; t < ntasks; t++) {
-    lpk[t] = (char)(basis_sets[atom_kinds[iatom_list[t] - 1] - 1]->lmax[iset_list[t] - 1] +
+    lak[t] = (char)basis_sets[atom_kinds[iatom_list[t] - 1] - 1]->lmax[iset_list[t] - 1];
+    lpk[t] = (char)(lak[t] +
                     basis_sets[atom_kinds[jatom_list[t] - 1] - 1]->lmax[jset_list[t] - 1]);
     if (2.0 * radius_list[t] < level_hmax[level_list[t] - 1])
       cls[t] = 2;
@@ -443,6 +447,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     if (cls[a] != cls[b]) return cls[a] < cls[b];
     if (level_list[a] != level_list[b]) return level_list[a] < level_list[b];
     if (lpk[a] != lpk[b]) return lpk[a] < lpk[b];
+    if (lak[a] != lak[b]) return lak[a] < lak[b];
     if (block_num_list[a] != block_num_list[b]) return block_num_list[a] < block_num_list[b];
     if (iset_list[a] != iset_list[b]) return iset_list[a] < iset_list[b];
     return jset_list[a] < jset_list[b];
@@ -717,6 +722,10 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->olev.assign(nlevels, OrthoLevel{});
   tl->lp0s.resize(ntasks);
   for (int s = 0; s < ntasks; s++) tl->lp0s[s] = (signed char)(tasks[s].la_max + tasks[s].lb_max);
+  tl->highl_before.assign(ntasks + 1, 0);
+  for (int s = 0; s < ntasks; s++)
+    tl->highl_before[s + 1] =
+        tl->highl_before[s] + (tasks[s].la_max > 2 || tasks[s].lb_max > 2 ? 1 : 0);
   for (int s = 0; s < ntasks; s++) {
     const Task &T = tasks[s];
     if (!(T.flags & kTaskOrtho)) continue;
@@ -1002,7 +1011,8 @@ int cxyz_stride(const TaskListImpl *tl, int sh) {
 
 // Workgroups of the tile kernels for lp shift sh: the items of each tile in
 // an lp bucket are dealt to a number of workgroups proportional to their
-// count, about one workgroup per CU over a level.
+// count, about four workgroups per CU over a level (one is resident per CU;
+// several rounds keep the last one short).
 TilePlan &tile_plan(TaskListImpl *tl, int sh) {
   auto it = tl->tile_plans.find(sh);
   if (it != tl->tile_plans.end()) return it->second;
@@ -1018,7 +1028,8 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh) {
         if (t.level == l) total += t.start[v1] - t.start[v0];
       const int first = (int)units.size();
       if (total > 0) {
-        const long long per = std::max<long long>(32, (total + num_cus() - 1) / num_cus());
+        const long long per =
+            std::max<long long>(32, (total + 4LL * num_cus() - 1) / (4LL * num_cus()));
         for (const TileInfo &t : tl->tiles) {
           if (t.level != l) continue;
           const int a = t.start[v0], b = t.start[v1];
@@ -1056,7 +1067,13 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanSetup);
       if (mode == kCollocate) {
         TimedLaunch tm(6, s);
-        HIP_CHECK(launch_cxyz_wave(Q, s));
+        // GRID_FUNC_AB without l shifts: lane-per-task setup for la, lb <= 2
+        const bool ab = P.func == 100 && P.ld[0] == 0 && P.ld[1] == 0 && P.ld[2] == 0 &&
+                        P.ld[3] == 0;
+        Q.lane_cxyz = ab ? 1 : 0;
+        if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
+        if (!ab || tl->highl_before[c2] > tl->highl_before[o.begin])
+          HIP_CHECK(launch_cxyz_wave(Q, s));
       }
       const int cuts[2][2] = {{o.begin, c1}, {c1, c2}};
       for (int r = 0; r < 2; r++) {

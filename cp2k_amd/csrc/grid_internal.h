@@ -157,6 +157,7 @@ struct PointParams {
   const TileUnit *units;  // collocation tiles
   const TileItem *items;
   int unit_base;          // first unit of the launch (one unit per workgroup)
+  int lane_cxyz;          // cab -> cxyz of GRID_FUNC_AB tasks with la, lb <= 2 by lane
 };
 
 }  // namespace gm

@@ -443,6 +443,108 @@ __device__ void wave_finish(const PointParams &P, const Task &T, const Group &G,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Lane-per-task cab -> cxyz for GRID_FUNC_AB (cab = the density sub-block),
+// la, lb <= 2: every loop over orbitals, binomials and (lx, ly, lz) is
+// unrolled at compile time; accumulators live in registers.
+// (x - a)^ax (x - b)^bx = sum_l alpha[l] (x - p)^l with
+// alpha[k + m] += C(ax, k) C(bx, m) (p - a)^(ax - k) (p - b)^(bx - m),
+// src/grid/ref/grid_ref_collint.h:840-861.
+// ---------------------------------------------------------------------------
+constexpr OrbTable kOrb = make_orb_table();
+
+constexpr int binom(int n, int k) {
+  int r = 1;
+  for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+  return r;
+}
+
+constexpr int ncoset_c(int l) { return l < 0 ? 0 : (l + 1) * (l + 2) * (l + 3) / 6; }
+
+constexpr int pidx_c(int lx, int ly, int lz, int lp) {
+  int off = 0;
+  for (int z = 0; z < lz; z++) off += (lp - z + 1) * (lp - z + 2) / 2;
+  for (int y = 0; y < ly; y++) off += lp - lz - y + 1;
+  return off + lx;
+}
+
+// alpha of one axis for exponents (A, B): coefficients of (x - p)^l, l <= A + B
+template <int A, int B>
+__device__ __forceinline__ void axis_alpha(const double *pa, const double *pb, double *al) {
+#pragma unroll
+  for (int l = 0; l <= A + B; l++) al[l] = 0.0;
+#pragma unroll
+  for (int k = 0; k <= A; k++)
+#pragma unroll
+    for (int m = 0; m <= B; m++)
+      al[k + m] = fma((double)(binom(A, k) * binom(B, m)), pa[A - k] * pb[B - m], al[k + m]);
+}
+
+template <int LA, int LB, int IA, int IB>
+__device__ __forceinline__ void lane_pair(const double (&pa)[3][LA + 1], const double (&pb)[3][LB + 1],
+                                          double c, double *acc) {
+  constexpr int LP = LA + LB;
+  constexpr int AX = kOrb.v[IA][0], AY = kOrb.v[IA][1], AZ = kOrb.v[IA][2];
+  constexpr int BX = kOrb.v[IB][0], BY = kOrb.v[IB][1], BZ = kOrb.v[IB][2];
+  double ax[AX + BX + 1], ay[AY + BY + 1], az[AZ + BZ + 1];
+  axis_alpha<AX, BX>(pa[0], pb[0], ax);
+  axis_alpha<AY, BY>(pa[1], pb[1], ay);
+  axis_alpha<AZ, BZ>(pa[2], pb[2], az);
+#pragma unroll
+  for (int lz = 0; lz <= AZ + BZ; lz++)
+#pragma unroll
+    for (int ly = 0; ly <= AY + BY; ly++) {
+      const double cyz = c * (ay[ly] * az[lz]);
+#pragma unroll
+      for (int lx = 0; lx <= AX + BX; lx++)
+        acc[pidx_c(lx, ly, lz, LP)] = fma(cyz, ax[lx], acc[pidx_c(lx, ly, lz, LP)]);
+    }
+}
+
+template <int LA, int LB, int IA, int IB>
+struct PairLoop {
+  static __device__ __forceinline__ void run(const Task &T, const Group &G, const double *cart,
+                                             long long cart_len, double pref,
+                                             const double (&pa)[3][LA + 1],
+                                             const double (&pb)[3][LB + 1], double *acc) {
+    constexpr int la_i = kOrb.v[IA][0] + kOrb.v[IA][1] + kOrb.v[IA][2];
+    constexpr int lb_i = kOrb.v[IB][0] + kOrb.v[IB][1] + kOrb.v[IB][2];
+    if (la_i >= T.la_min && lb_i >= T.lb_min) {
+      const long long q = G.cart_off + (long long)(T.o2 + IB) * G.ncoa + T.o1 + IA;
+      const double p = (q < cart_len) ? cart[q] : 0.0;
+      if (p != 0.0) lane_pair<LA, LB, IA, IB>(pa, pb, p * pref, acc);
+    }
+    if constexpr (IA + 1 < ncoset_c(LA))
+      PairLoop<LA, LB, IA + 1, IB>::run(T, G, cart, cart_len, pref, pa, pb, acc);
+    else if constexpr (IB + 1 < ncoset_c(LB))
+      PairLoop<LA, LB, 0, IB + 1>::run(T, G, cart, cart_len, pref, pa, pb, acc);
+  }
+};
+
+template <int LA, int LB>
+__device__ __forceinline__ void lane_cxyz(const PointParams &P, const Task &T, const Group &G,
+                                          double *out) {
+  constexpr int LP = LA + LB;
+  constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
+  double acc[NC];
+#pragma unroll
+  for (int e = 0; e < NC; e++) acc[e] = 0.0;
+  double pa[3][LA + 1], pb[3][LB + 1];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double da = T.rp[d] - T.ra[d], db = T.rp[d] - T.rb[d];
+    pa[d][0] = 1.0;
+    pb[d][0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= LA; k++) pa[d][k] = pa[d][k - 1] * da;
+#pragma unroll
+    for (int k = 1; k <= LB; k++) pb[d][k] = pb[d][k - 1] * db;
+  }
+  PairLoop<LA, LB, 0, 0>::run(T, G, P.cart, P.cart_len, T.rscale * T.pexp, pa, pb, acc);
+#pragma unroll
+  for (int e = 0; e < NC; e++) out[e] = acc[e];
+}
+
 struct TaskL {
   int la, lb, la_min, lb_min, lp;
 };
@@ -464,7 +566,32 @@ __device__ __forceinline__ TaskL task_l(const PointParams &P, const Task &T) {
 // belong to a geometry class, with lp <= LPB (point kernels).
 // ---------------------------------------------------------------------------
 
-// cab -> cxyz of one task per wave, stored packed at cxyz_all[task * stride].
+// cab -> cxyz of one task per lane (GRID_FUNC_AB, la_max, lb_max <= 2).
+__global__ __launch_bounds__(256) void cxyz_lane_kernel(const Task *__restrict__ tasks,
+                                                        const Group *__restrict__ groups,
+                                                        PointParams P) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.ntasks) return;
+  const long long task = P.task_base + t;
+  const Task &T = tasks[task];
+  if (T.la_max > 2 || T.lb_max > 2) return;  // cxyz_wave_kernel takes these
+  const Group &G = groups[T.group];
+  double *out = P.cxyz_all + task * P.cxyz_stride;
+  switch (T.la_max * 3 + T.lb_max) {
+    case 0: lane_cxyz<0, 0>(P, T, G, out); break;
+    case 1: lane_cxyz<0, 1>(P, T, G, out); break;
+    case 2: lane_cxyz<0, 2>(P, T, G, out); break;
+    case 3: lane_cxyz<1, 0>(P, T, G, out); break;
+    case 4: lane_cxyz<1, 1>(P, T, G, out); break;
+    case 5: lane_cxyz<1, 2>(P, T, G, out); break;
+    case 6: lane_cxyz<2, 0>(P, T, G, out); break;
+    case 7: lane_cxyz<2, 1>(P, T, G, out); break;
+    default: lane_cxyz<2, 2>(P, T, G, out); break;
+  }
+}
+
+// cab -> cxyz of one task per wave, stored packed at cxyz_all[task * stride];
+// with lane_cxyz set, tasks the lane kernel takes are skipped.
 __global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__ tasks,
                                                         const Group *__restrict__ groups,
                                                         PointParams P) {
@@ -475,6 +602,7 @@ __global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__
   if (t >= P.ntasks) return;
   const long long task = P.task_base + t;
   const Task &T = tasks[task];
+  if (P.lane_cxyz && T.la_max <= 2 && T.lb_max <= 2) return;
   const TaskL l = task_l(P, T);
   const WaveLds W = wave_lds(lds, P.wp, wave);
   const int NP = P.wp.np;
@@ -644,6 +772,13 @@ static hipError_t launch_wave(K kernel, const PointParams &P, int blocks, hipStr
 }
 
 static int wave_blocks(const PointParams &P) { return (P.ntasks + P.wp.nwaves - 1) / P.wp.nwaves; }
+
+hipError_t launch_cxyz_lane(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cxyz_lane_kernel, dim3((P.ntasks + 255) / 256), dim3(256), 0, s, P.tasks,
+                     P.groups, P);
+  return hipGetLastError();
+}
 
 hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
