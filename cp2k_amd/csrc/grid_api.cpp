@@ -1101,6 +1101,9 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       }
       if (mode == kIntegrate) {
         TimedLaunch tm(10, s);
+        // one wave per task: its hab atomics are contiguous per task (a
+        // task per lane scatters them over 64 blocks and measured slower)
+        Q.lane_cxyz = 0;
         HIP_CHECK(launch_finish_wave(Q, s));
       }
     }

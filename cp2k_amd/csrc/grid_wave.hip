@@ -258,6 +258,82 @@ __device__ __forceinline__ void collocate_columns(const PointParams &P, const Ta
   }
 }
 
+// D[lz] = sum_ly py[ly] sum_lx cxyz[(lz, ly, lx)] px[lx] of one column.
+template <int LP>
+__device__ __forceinline__ void column_poly(const double *cx, const double *px, const double *py,
+                                            double *D) {
+  int e = 0;
+#pragma unroll
+  for (int lz = 0; lz <= LP; lz++) {
+    double s = 0.0;
+#pragma unroll
+    for (int ly = 0; ly <= LP - lz; ly++) {
+      double t = 0.0;
+#pragma unroll
+      for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fma(cx[e], px[lx], t);
+      s = fma(t, py[ly], s);
+    }
+    D[lz] = s;
+  }
+}
+
+// NCL groups of 64 columns at once (columns c0 + 64 u + lane, u < NCL): the
+// planes loop, its pz reads and its scalar bookkeeping serve NCL points per
+// lane and step.
+template <int LP, int NCL>
+__device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, const GeoClass &G,
+                                           const Layout &lay, const WavePlan &wp,
+                                           const double *tab, const double *cx,
+                                           const TileItem &I, int gy0, int gz0, const int cw[3],
+                                           int c0, int c_end, double *win, int lane) {
+  const int NP = wp.np;
+  const double *tabz = tab + 2 * wp.ext * NP;
+  const int nx = lay.npts_local[0];
+  const int nxby = nx * wp.by;
+  const int ngz = lay.npts_global[2];
+  Column C[NCL];
+  int lbase[NCL];
+  unsigned span[NCL];
+  double D[NCL][LP + 1];
+  int klmin = 1;
+#pragma unroll
+  for (int u = 0; u < NCL; u++) {
+    const int c = c0 + 64 * u + lane;
+    C[u] = load_column(P, G, lay, cw, c < c_end ? c : G.ncol);
+    lbase[u] = 0;
+    if (C[u].kl <= 0) {
+      const int ly = C[u].base / nx - gy0;  // base = y * nx + x
+      if ((unsigned)ly < (unsigned)wp.by)
+        lbase[u] = C[u].base - gy0 * nx;
+      else
+        C[u].kl = 1;  // outside the tile: never for items built on the host
+    }
+    double px[LP + 1], py[LP + 1];
+    load_pxy<LP>(T, wp, tab, C[u], px, py);
+    column_poly<LP>(cx, px, py, D[u]);
+    span[u] = (unsigned)(2 - 2 * C[u].kl);  // 0 for an idle lane
+    klmin = min(klmin, C[u].kl);
+  }
+  const int kmin = __builtin_amdgcn_readfirstlane(wave_min(klmin));
+  const int ka = max(kmin, (int)I.k0), kb = min(1 - kmin, (int)I.k1);
+  int gz = wrap(cw[2] + ka, ngz);
+#pragma unroll 2
+  for (int k = ka; k <= kb; k++) {
+    const double *pz = tabz + (k - T.lo[2]) * NP;
+    const int tz = gz - gz0;
+    if ((unsigned)tz < (unsigned)wp.bz) {
+#pragma unroll
+      for (int u = 0; u < NCL; u++) {
+        double v = 0.0;
+#pragma unroll
+        for (int lz = 0; lz <= LP; lz++) v = fma(D[u][lz], pz[lz], v);
+        if ((unsigned)(k - C[u].kl) < span[u]) atomicAdd(&win[tz * nxby + lbase[u]], v);
+      }
+    }
+    gz = (gz + 1 == ngz) ? 0 : gz + 1;
+  }
+}
+
 // Collocate the rows j0..j1 and planes k0..k1 of a task into an LDS tile
 // win[(z - gz0) * nx * by + (y - gy0) * nx + x].
 template <int LP>
@@ -266,98 +342,88 @@ __device__ __forceinline__ void tile_columns(const PointParams &P, const Task &T
                                              const WavePlan &wp, const double *tab,
                                              const double *cx, const TileItem &I, int gy0,
                                              int gz0, double *win, int lane) {
-  const int NP = wp.np;
-  const double *tabz = tab + 2 * wp.ext * NP;
-  const int nx = lay.npts_local[0];
-  const int nxby = nx * wp.by;
-  const int ngz = lay.npts_global[2];
   int cw[3];
   axis_offsets(T, lay, cw);
   const int c_beg = P.geo_rows[G.row_off + (I.j0 - G.lo[1])];
   const int c_end = P.geo_rows[G.row_off + (I.j1 + 1 - G.lo[1])];
-  for (int c0 = c_beg; c0 < c_end; c0 += 64) {
-    const int c = c0 + lane;
-    Column C = load_column(P, G, lay, cw, c < c_end ? c : G.ncol);
-    int lbase = 0;
-    if (C.kl <= 0) {
-      const int ly = C.base / nx - gy0;  // base = y * nx + x
-      if ((unsigned)ly < (unsigned)wp.by)
-        lbase = C.base - gy0 * nx;
-      else
-        C.kl = 1;  // outside the tile: never for items built on the host
-    }
-    double px[LP + 1], py[LP + 1], D[LP + 1];
-    load_pxy<LP>(T, wp, tab, C, px, py);
-    int e = 0;
-#pragma unroll
-    for (int lz = 0; lz <= LP; lz++) {
-      double s = 0.0;
-#pragma unroll
-      for (int ly = 0; ly <= LP - lz; ly++) {
-        double t = 0.0;
-#pragma unroll
-        for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fma(cx[e], px[lx], t);
-        s = fma(t, py[ly], s);
-      }
-      D[lz] = s;
-    }
-    const int kmin = __builtin_amdgcn_readfirstlane(wave_min(C.kl));
-    const int ka = max(kmin, (int)I.k0), kb = min(1 - kmin, (int)I.k1);
-    const unsigned span = (unsigned)(2 - 2 * C.kl);  // 0 for an idle lane
-    int gz = wrap(cw[2] + ka, ngz);
-#pragma unroll 2
-    for (int k = ka; k <= kb; k++) {
-      const double *pz = tabz + (k - T.lo[2]) * NP;
-      double v = 0.0;
-#pragma unroll
-      for (int lz = 0; lz <= LP; lz++) v = fma(D[lz], pz[lz], v);
-      const int tz = gz - gz0;
-      if ((unsigned)tz < (unsigned)wp.bz && (unsigned)(k - C.kl) < span)
-        atomicAdd(&win[tz * nxby + lbase], v);
-      gz = (gz + 1 == ngz) ? 0 : gz + 1;
+  for (int c0 = c_beg; c0 < c_end;) {
+    if (LP <= 4 && c_end - c0 > 64) {  // two columns per lane fit the registers for lp <= 4
+      tile_group<LP, (LP <= 4 ? 2 : 1)>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
+      c0 += (LP <= 4 ? 128 : 64);
+    } else {
+      tile_group<LP, 1>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
+      c0 += 64;
     }
   }
 }
 
-// Integrate the task's points into lane-private accumulators acc[(lz, ly, lx)].
-template <int LP>
-__device__ __forceinline__ void integrate_columns(const PointParams &P, const Task &T, const GeoClass &G,
-                                  const Layout &lay, const WavePlan &wp, const double *tab,
-                                  const double *grid, double *acc, int lane) {
+// Integrate NCL groups of 64 columns at once into acc[(lz, ly, lx)].
+template <int LP, int NCL>
+__device__ __forceinline__ void integrate_group(const PointParams &P, const Task &T,
+                                                const GeoClass &G, const Layout &lay,
+                                                const WavePlan &wp, const double *tab,
+                                                const double *grid, const int cw[3], int c0,
+                                                double *acc, int lane) {
   const int NP = wp.np;
   const double *tabz = tab + 2 * wp.ext * NP;
   const int nxy = lay.npts_local[0] * lay.npts_local[1];
   const int ngz = lay.npts_global[2], nz = lay.npts_local[2];
-  int cw[3];
-  axis_offsets(T, lay, cw);
-  for (int c0 = 0; c0 < G.ncol; c0 += 64) {
-    const Column C = load_column(P, G, lay, cw, c0 + lane);
-    double px[LP + 1], py[LP + 1], M[LP + 1];
-    load_pxy<LP>(T, wp, tab, C, px, py);
+  Column C[NCL];
+  unsigned span[NCL];
+  double M[NCL][LP + 1];
+  int kmin = 1;
 #pragma unroll
-    for (int l = 0; l <= LP; l++) M[l] = 0.0;
-    const int kmin = P.geo_kmin[G.grp_off + (c0 >> 6)];
-    const unsigned span = (unsigned)(2 - 2 * C.kl);
-    int gz = wrap(cw[2] + kmin, ngz);
-#pragma unroll 2
-    for (int k = kmin; k <= 1 - kmin; k++) {
-      const double *pz = tabz + (k - T.lo[2]) * NP;
-      double g = 0.0;
-      if (gz < nz && (unsigned)(k - C.kl) < span) g = grid[gz * nxy + C.base];
+  for (int u = 0; u < NCL; u++) {
+    C[u] = load_column(P, G, lay, cw, c0 + 64 * u + lane);
+    span[u] = (unsigned)(2 - 2 * C[u].kl);
+    kmin = min(kmin, P.geo_kmin[G.grp_off + (c0 >> 6) + u]);
 #pragma unroll
-      for (int l = 0; l <= LP; l++) M[l] = fma(g, pz[l], M[l]);
-      gz = (gz + 1 == ngz) ? 0 : gz + 1;
+    for (int l = 0; l <= LP; l++) M[u][l] = 0.0;
+  }
+  int gz = wrap(cw[2] + kmin, ngz);
+#pragma unroll 4  // several grid loads in flight per wave
+  for (int k = kmin; k <= 1 - kmin; k++) {
+    const double *pz = tabz + (k - T.lo[2]) * NP;
+    double g[NCL];
+#pragma unroll
+    for (int u = 0; u < NCL; u++) {
+      g[u] = 0.0;
+      if (gz < nz && (unsigned)(k - C[u].kl) < span[u]) g[u] = grid[gz * nxy + C[u].base];
     }
+#pragma unroll
+    for (int u = 0; u < NCL; u++)
+#pragma unroll
+      for (int l = 0; l <= LP; l++) M[u][l] = fma(g[u], pz[l], M[u][l]);
+    gz = (gz + 1 == ngz) ? 0 : gz + 1;
+  }
+#pragma unroll
+  for (int u = 0; u < NCL; u++) {
+    double px[LP + 1], py[LP + 1];
+    load_pxy<LP>(T, wp, tab, C[u], px, py);
     int e = 0;
 #pragma unroll
     for (int lz = 0; lz <= LP; lz++)
 #pragma unroll
       for (int ly = 0; ly <= LP - lz; ly++) {
-        const double yz = py[ly] * M[lz];
+        const double yz = py[ly] * M[u][lz];
 #pragma unroll
         for (int lx = 0; lx <= LP - lz - ly; lx++, e++) acc[e] = fma(px[lx], yz, acc[e]);
       }
   }
+}
+
+// Integrate the task's points into lane-private accumulators acc[(lz, ly, lx)].
+template <int LP>
+__device__ __forceinline__ void integrate_columns(const PointParams &P, const Task &T,
+                                                  const GeoClass &G, const Layout &lay,
+                                                  const WavePlan &wp, const double *tab,
+                                                  const double *grid, double *acc, int lane) {
+  int cw[3];
+  axis_offsets(T, lay, cw);
+  // one column per lane: two (integrate_group<LP, 2>) cost occupancy (156
+  // VGPRs) and measured slower on h2o64
+  for (int c0 = 0; c0 < G.ncol; c0 += 64)
+    integrate_group<LP, 1>(P, T, G, lay, wp, tab, grid, cw, c0, acc, lane);
 }
 
 template <int LP>
