@@ -153,6 +153,22 @@ def rocprof_summary(d, calls, copy_mib):
             tb = (fk * ff + wk * wf) * 1024.0
             out.append(f"call {call}: fetch {fk:.0f} KiB, write {wk:.0f} KiB, "
                        f"traffic_bytes {tb:.0f}")
+    # any other counters: totals per kernel (all dispatches of the run)
+    kern = {}
+    for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        for r in csv.DictReader(open(f)):
+            c = r["Counter_Name"]
+            if c in ("FETCH_SIZE", "WRITE_SIZE"):
+                continue
+            name = re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", "")
+            kern.setdefault(name, {})
+            kern[name][c] = kern[name].get(c, 0.0) + float(r["Counter_Value"])
+    if kern:
+        out.append(f"== counters per kernel, totals over the run ({calls} call(s))")
+        for name in sorted(kern):
+            out.append(name)
+            for c in sorted(kern[name]):
+                out.append(f"    {c:26s} {kern[name][c]:14.5g}")
     return "\n".join(out)
 
 
