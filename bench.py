@@ -65,6 +65,8 @@ def parse():
     p.add_argument("--precision", choices=["fp64", "fp32mixed"], default="fp64")
     p.add_argument("--forces", action="store_true", help="integrate also forces+virial")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-fp32", action="store_true",
+                   help="skip the FP32-mixed leg (throughput and per-result errors)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="before the steps, run 3 device-to-device copies of MIB MiB: "
@@ -233,6 +235,66 @@ def cpu_baseline(args, pab, V, seconds):
                        f"collocate {t1 - t0:.2f} s, integrate {t2 - t1:.2f} s")}
 
 
+def timed_steps(step, steps):
+    from cp2k_amd import grid
+    grid.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    grid.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def outputs(gt, pab_d, rho_d, V_d, natoms, buffer_size):
+    """One collocate and one integrate with forces and virial, on the
+    device, copied to the host: (rho levels, hab, forces, virial)."""
+    from cp2k_amd import grid
+    hab = grid.DeviceArray(buffer_size)
+    f = grid.DeviceArray((natoms, 3))
+    v = grid.DeviceArray((3, 3))
+    gt.collocate_dev(100, pab_d, rho_d)
+    gt.integrate_dev(V_d, hab, pab=pab_d, forces=f, virial=v)
+    grid.synchronize()
+    res = ([r.numpy() for r in rho_d], hab.numpy(), f.numpy(), v.numpy())
+    for a in (hab, f, v):
+        a.free()
+    return res
+
+
+def fp32_leg(opt, gt, step, pab_d, rho_d, V_d, points, ms_fp64):
+    """The same steps in FP32-mixed mode, and the error of its outputs
+    (rho per level, hab, forces, virial) against the FP64 outputs."""
+    from cp2k_amd import grid, precision
+    args = gt.args
+    grid.set_precision(grid.FP64)
+    ref = outputs(gt, pab_d, rho_d, V_d, args.natoms, args.buffer_size)
+    grid.set_precision(grid.FP32_MIXED)
+    for _ in range(opt.warmup):
+        step()
+    grid.reset_timing()
+    grid.set_timing(True)
+    ms = timed_steps(step, opt.steps)
+    grid.set_timing(False)
+    kt = {k: grid.kernel_time_ms(i) for i, k in enumerate(grid.KERNEL_NAMES)}
+    got = outputs(gt, pab_d, rho_d, V_d, args.natoms, args.buffer_size)
+    grid.set_precision(grid.FP64)
+    err = precision.report(rho=(got[0], ref[0]), hab=(got[1], ref[1]),
+                           forces=(got[2], ref[2]), virial=(got[3], ref[3]))
+    return {
+        "value": round(2.0 * points / (ms * 1e-3) / 1e9, 3),
+        "unit": "Gpts/s",
+        "ms_per_step": round(ms, 3),
+        "speedup_vs_fp64": round(ms_fp64 / ms, 3),
+        "kernel_ms_per_step": {k: round(v[0] / max(opt.steps, 1), 4) for k, v in kt.items()},
+        "errors_vs_fp64": {k: (round(v, 12) if isinstance(v, float) else
+                               [{a: round(b, 12) for a, b in e.items()} for e in v])
+                           for k, v in err.items()},
+        "bars": {"rho_linf": precision.RHO_LINF_BAR, "forces_linf": precision.FORCE_BAR},
+        "within_bars": bool(err["rho_linf"] <= precision.RHO_LINF_BAR and
+                            err["forces_linf"] <= precision.FORCE_BAR),
+    }
+
+
 def main():
     opt = parse()
     if opt.rocprof_summary:
@@ -343,6 +405,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not opt.no_fp32:
+        out["fp32_mixed"] = fp32_leg(opt, gt, step, pab_d, rho_d, V_d, points, ms_step)
     if not opt.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, pab, V, opt.cpu_seconds)
     out["config"]["generate_s"] = round(gen_s, 2)

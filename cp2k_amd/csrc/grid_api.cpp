@@ -37,6 +37,7 @@ hipError_t launch_contract(const Group *groups, int ngroups, const double *hab_c
                            const double *sphi, double *hab, long long hab_len, int cach,
                            int max_nsb, int max_nsa_nsb, hipStream_t s);
 hipError_t launch_fill(double *p, long long n, double v, hipStream_t s);
+hipError_t launch_to_float(const double *src, float *dst, long long n, hipStream_t s);
 hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *task_out,
                                 double *forces, int natoms, hipStream_t s);
 hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
@@ -81,7 +82,7 @@ static inline int pmod(int a, int m) { return ((a % m) + m) % m; }
 // Library state (src/grid/common/grid_library.c)
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kNumKernelIds = 12;
+constexpr int kNumKernelIds = 13;
 constexpr int kCounterLp = 20;
 
 struct PendingTiming {
@@ -288,12 +289,24 @@ struct TaskListImpl {
   std::vector<Layout> layouts;
   std::vector<int> npts_local;  // [nlevels][3]
   // device-resident description
-  DevBuf d_tasks, d_groups, d_sphi, d_fptr, d_fent, d_levels, d_cart, d_hab_cart, d_task_out;
+  DevBuf d_tasks, d_groups, d_sphi, d_fptr, d_fent, d_cart, d_hab_cart, d_task_out;
   DevBuf d_geo, d_geo_cols, d_geo_kmin, d_cxyz;  // geometry classes, per-task cxyz
   DevBuf d_geo_rows, d_items;                      // row starts, collocation tile items
+  DevBuf d_gridf;                                  // FP32-mixed: potential rounded to FP32
   std::vector<TileInfo> tiles;
   std::map<int, TilePlan> tile_plans;              // by lp shift
-  Level *h_levels = nullptr;  // pinned staging for the per-call level table
+  // Per-call level tables.  A call returns once its work is enqueued, so the
+  // table of a call still in flight must not be rewritten: each call takes
+  // the next slot of a ring (pinned staging + device copy), and waits for
+  // the copy of that slot's previous call before writing it.
+  static constexpr int kLevelSlots = 8;
+  struct LevelSlot {
+    Level *h = nullptr;
+    DevBuf d;
+    hipEvent_t copied = nullptr;
+  };
+  LevelSlot level_slots[kLevelSlots];
+  int next_level_slot = 0, last_level_slot = 0;
   long long cart_len = 0;
   // maxima for the LDS plan
   int max_la = 0, max_lb = 0, max_lp = 0, max_cmax = 0, max_rows_ortho = 0;
@@ -316,13 +329,19 @@ struct TaskListImpl {
   std::vector<DevBuf> grid_dev;
 
   ~TaskListImpl() {
-    DevBuf *bufs[] = {&d_tasks, &d_groups, &d_sphi, &d_fptr, &d_fent, &d_levels,
+    DevBuf *bufs[] = {&d_tasks, &d_groups, &d_sphi, &d_fptr, &d_fent,
                       &d_cart, &d_hab_cart, &d_task_out, &pab_dev, &hab_dev, &forces_dev,
-                      &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz, &d_geo_rows, &d_items};
+                      &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz, &d_geo_rows, &d_items,
+                      &d_gridf};
     for (auto &kv : tile_plans) kv.second.d.release();
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
-    if (h_levels) (void)hipHostFree(h_levels);
+    for (LevelSlot &ls : level_slots) {
+      if (ls.copied) (void)hipEventSynchronize(ls.copied);
+      if (ls.copied) (void)hipEventDestroy(ls.copied);
+      if (ls.h) (void)hipHostFree(ls.h);
+      ls.d.release();
+    }
     magic = 0;
   }
 };
@@ -856,8 +875,6 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   upload(tl->d_geo_kmin, geo_kmin);
   upload(tl->d_geo_rows, geo_rows);
   upload(tl->d_items, items);
-  tl->d_levels.get(sizeof(Level) * nlevels);
-  if (!tl->h_levels) HIP_CHECK(hipHostMalloc((void **)&tl->h_levels, sizeof(Level) * 64));
   if (nlevels > 64) fatal("at most 64 grid levels are supported");
   tl->d_cart.get(sizeof(double) * std::max<long long>(tl->cart_len, 1));
   tl->d_hab_cart.get(sizeof(double) * std::max<long long>(tl->cart_len, 1));
@@ -1153,15 +1170,42 @@ void set_wave_buffers(TaskListImpl *tl, PointParams &P) {
                                         (size_t)std::max(tl->ntasks, 1));
 }
 
-void set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s) {
+long long level_points(const TaskListImpl *tl, int l) {
+  return (long long)tl->npts_local[3 * l] * tl->npts_local[3 * l + 1] * tl->npts_local[3 * l + 2];
+}
+
+// Level table of a call (device pointer); with `gridf`, level l of the
+// potential rounded to FP32 sits at gridf + (points of the levels before l).
+const Level *set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s,
+                        const float *gridf = nullptr) {
+  TaskListImpl::LevelSlot &ls = tl->level_slots[tl->next_level_slot];
+  tl->next_level_slot = (tl->next_level_slot + 1) % TaskListImpl::kLevelSlots;
+  if (!ls.h) {
+    HIP_CHECK(hipHostMalloc((void **)&ls.h, sizeof(Level) * 64));
+    HIP_CHECK(hipEventCreateWithFlags(&ls.copied, hipEventDisableTiming));
+  } else {
+    HIP_CHECK(hipEventSynchronize(ls.copied));  // the slot's previous copy is done
+  }
+  ls.d.get(sizeof(Level) * 64);
+  long long off = 0;
   for (int l = 0; l < tl->nlevels; l++) {
-    Level &L = tl->h_levels[l];
+    Level &L = ls.h[l];
     L.lay = tl->layouts[l];
     L.grid = grids[l];
     L.npts = (long long)L.lay.npts_local[0] * L.lay.npts_local[1] * L.lay.npts_local[2];
+    L.gridf = gridf ? gridf + off : nullptr;
+    off += L.npts;
   }
-  HIP_CHECK(hipMemcpyAsync(tl->d_levels.p, tl->h_levels, sizeof(Level) * tl->nlevels,
-                           hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(ls.d.p, ls.h, sizeof(Level) * tl->nlevels, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipEventRecord(ls.copied, s));
+  tl->last_level_slot = (int)(&ls - tl->level_slots);
+  return (const Level *)ls.d.p;
+}
+
+// After the last kernel of a call: the slot is free once that kernel is done
+// (also when the next call of the ring comes on another stream).
+void end_levels(TaskListImpl *tl, hipStream_t s) {
+  HIP_CHECK(hipEventRecord(tl->level_slots[tl->last_level_slot].copied, s));
 }
 
 void count_stats(const TaskListImpl *tl, bool collocate, int lpshift) {
@@ -1192,23 +1236,22 @@ void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const 
                    hipStream_t s) {
   int ld[4];
   if (!func_ldiffs(func, ld)) fatal("unknown grid_func %d", func);
-  if (precision() != GRID_MI355X_FP64)
-    fatal("FP32-mixed collocation is not available in this build");
-  set_levels(tl, grids, s);
+  const Level *levels = set_levels(tl, grids, s);
   {
     TimedLaunch tm(4, s);
     for (int l = 0; l < tl->nlevels; l++)
-      HIP_CHECK(launch_fill(grids[l], tl->h_levels[l].npts, 0.0, s));
+      HIP_CHECK(launch_fill(grids[l], level_points(tl, l), 0.0, s));
   }
   run_decontract(tl, pab, s);
   PointParams P{};
   P.tasks = (const Task *)tl->d_tasks.p;
   P.ntasks = tl->ntasks;
   P.groups = (const Group *)tl->d_groups.p;
-  P.levels = (const Level *)tl->d_levels.p;
+  P.levels = levels;
   P.cart = (const double *)tl->d_cart.p;
   P.cart_len = tl->cart_len;
   P.func = func;
+  P.fp32 = precision() == GRID_MI355X_FP32_MIXED;
   for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
   P.plan = make_plan(tl, ld, false);
   set_wave_buffers(tl, P);
@@ -1216,6 +1259,7 @@ void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const 
     TimedLaunch tm(0, s);
     launch_ranges(tl, P, s, kCollocate);
   }
+  end_levels(tl, s);
   count_stats(tl, true, ld[0] + ld[2]);
 }
 
@@ -1226,11 +1270,22 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   if (do_v && !do_f)
     fatal("virial requires forces (src/grid/ref/grid_ref_integrate.c:109)");
   if ((do_f || do_v) && !pab) fatal("forces/virial require pab_blocks");
-  if (precision() != GRID_MI355X_FP64)
-    fatal("FP32-mixed integration is not available in this build");
+  const bool fp32 = precision() == GRID_MI355X_FP32_MIXED;
   int ld[4];
   integrate_ldiffs(do_f, do_v, tau, ld);
-  set_levels(tl, (double *const *)grids, s);
+  float *gridf = nullptr;
+  if (fp32) {
+    long long total = 0;
+    for (int l = 0; l < tl->nlevels; l++) total += level_points(tl, l);
+    gridf = (float *)tl->d_gridf.get(sizeof(float) * (size_t)std::max(total, 1LL));
+    TimedLaunch tm(12, s);
+    long long off = 0;
+    for (int l = 0; l < tl->nlevels; l++) {
+      HIP_CHECK(launch_to_float(grids[l], gridf + off, level_points(tl, l), s));
+      off += level_points(tl, l);
+    }
+  }
+  const Level *levels = set_levels(tl, (double *const *)grids, s, gridf);
   if (do_f) run_decontract(tl, pab, s);
   {
     TimedLaunch tm(4, s);
@@ -1240,7 +1295,7 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   P.tasks = (const Task *)tl->d_tasks.p;
   P.ntasks = tl->ntasks;
   P.groups = (const Group *)tl->d_groups.p;
-  P.levels = (const Level *)tl->d_levels.p;
+  P.levels = levels;
   P.cart = (const double *)tl->d_cart.p;
   P.hab_cart = (double *)tl->d_hab_cart.p;
   P.task_out = (double *)tl->d_task_out.p;
@@ -1249,6 +1304,7 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   P.compute_tau = tau;
   P.do_forces = do_f;
   P.do_virial = do_v;
+  P.fp32 = fp32;
   P.plan = make_plan(tl, ld, true);
   set_wave_buffers(tl, P);
   if (do_f && tl->n_skip > 0) {  // skipped tasks contribute no forces
@@ -1280,6 +1336,7 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
     if (do_v)
       HIP_CHECK(launch_reduce_virial((const double *)tl->d_task_out.p, tl->ntasks, virial, s));
   }
+  end_levels(tl, s);
   count_stats(tl, false, ld[0] + ld[2]);
 }
 
@@ -1738,12 +1795,12 @@ long long grid_mi355x_task_points(const grid_task_list *task_list) {
   if (tl->points >= 0) return tl->points;
   const hipStream_t s = stream();
   std::vector<double *> g(tl->nlevels, nullptr);
-  set_levels(tl, g.data(), s);
+  const Level *levels = set_levels(tl, g.data(), s);
   PointParams P{};
   P.tasks = (const Task *)tl->d_tasks.p;
   P.ntasks = tl->ntasks;
   P.groups = (const Group *)tl->d_groups.p;
-  P.levels = (const Level *)tl->d_levels.p;
+  P.levels = levels;
   P.task_out = (double *)tl->d_task_out.p;
   P.count_only = 1;
   const int ld[4] = {0, 0, 0, 0};

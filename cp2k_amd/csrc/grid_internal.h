@@ -25,6 +25,7 @@ struct Layout {
 struct Level {
   Layout lay;
   double *grid;         // device pointer of this call's level grid
+  const float *gridf;   // FP32-mixed integrate: the potential rounded to FP32
   long long npts;       // nx*ny*nz of the local grid (bounds guard)
 };
 
@@ -158,6 +159,7 @@ struct PointParams {
   const TileItem *items;
   int unit_base;          // first unit of the launch (one unit per workgroup)
   int lane_cxyz;          // cab -> cxyz of GRID_FUNC_AB tasks with la, lb <= 2 by lane
+  int fp32;               // FP32-mixed: point evaluation in FP32, accumulation in FP64
 };
 
 }  // namespace gm

@@ -405,7 +405,8 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
           const double rx = di * lay.dh[0][0] + dj * lay.dh[1][0] + dk * lay.dh[2][0];
           const double ry = di * lay.dh[0][1] + dj * lay.dh[1][1] + dk * lay.dh[2][1];
           const double rz = di * lay.dh[0][2] + dj * lay.dh[1][2] + dk * lay.dh[2][2];
-          double dip = exp(-T.zetp * (rx * rx + ry * ry + rz * rz));
+          const double arg = -T.zetp * (rx * rx + ry * ry + rz * rz);
+          double dip = P.fp32 ? (double)__expf((float)arg) : exp(arg);
           for (int il = 0; il <= lp; il++) {
             v = fma(cr[il], dip, v);
             dip *= di;
@@ -590,7 +591,8 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
           const double rx = di * lay.dh[0][0] + dj * lay.dh[1][0] + dk * lay.dh[2][0];
           const double ry = di * lay.dh[0][1] + dj * lay.dh[1][1] + dk * lay.dh[2][1];
           const double rz = di * lay.dh[0][2] + dj * lay.dh[1][2] + dk * lay.dh[2][2];
-          double dip = g * exp(-T.zetp * (rx * rx + ry * ry + rz * rz));
+          const double arg = -T.zetp * (rx * rx + ry * ry + rz * rz);
+          double dip = g * (P.fp32 ? (double)__expf((float)arg) : exp(arg));
 #pragma unroll
           for (int l = 0; l <= LPB; l++)
             if (l <= lp) {
@@ -790,6 +792,13 @@ __global__ __launch_bounds__(256) void contract_kernel(const Group *groups,
   }
 }
 
+// FP32-mixed integrate: the potential rounded to FP32 once per call.
+__global__ void to_float_kernel(const double *src, float *dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
 __global__ void fill_kernel(double *p, long long n, double v) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -921,6 +930,14 @@ hipError_t launch_contract(const Group *groups, int ngroups, const double *hab_c
   const size_t lds = sizeof(double) * ((size_t)max_nsa_nsb + (size_t)cach * (size_t)max_nsb);
   hipLaunchKernelGGL(contract_kernel, dim3(ngroups), dim3(256), lds, s, groups, hab_cart,
                      sphi, hab, hab_len, cach);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_float(const double *src, float *dst, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(to_float_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
   return hipGetLastError();
 }
 

@@ -94,6 +94,10 @@ __device__ __forceinline__ int packed_index(int lx, int ly, int lz, int lp) {
   return off + lx;
 }
 
+// Multiply-add in the evaluation type: FP64, or FP32 in FP32-mixed mode.
+__device__ __forceinline__ double fmar(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // Per-wave LDS region.
 struct WaveLds {
   double *cxyz;  // dense [np][np][np] (setup, finish) or packed (collocate points)
@@ -108,9 +112,11 @@ __device__ __forceinline__ WaveLds wave_lds(double *lds, const WavePlan &wp, int
   return w;
 }
 
-// Per-axis tables, tab[(d * ext + c - lo_d) * np + l].
+// Per-axis tables, tab[(d * ext + c - lo_d) * np + l], computed in FP64 and
+// stored in the evaluation type R.
+template <typename R>
 __device__ void wave_tables(const Task &T, const Layout &lay, int lp, const WavePlan &wp,
-                            double *tab, int lane) {
+                            R *tab, int lane) {
   const int n0 = 2 - 2 * T.lo[0], n1 = 2 - 2 * T.lo[1], n2 = 2 - 2 * T.lo[2];
   for (int e = lane; e < n0 + n1 + n2; e += 64) {
     const int d = e < n0 ? 0 : (e < n0 + n1 ? 1 : 2);
@@ -121,9 +127,9 @@ __device__ void wave_tables(const Task &T, const Layout &lay, int lp, const Wave
     const int c = lo + q;
     const double t = c * h - roff;
     double v = exp(-T.zetp * t * t);
-    double *row = tab + (d * wp.ext + q) * wp.np;
+    R *row = tab + (d * wp.ext + q) * wp.np;
     for (int l = 0; l <= lp; l++) {
-      row[l] = v;
+      row[l] = (R)v;
       v *= t;
     }
   }
@@ -154,12 +160,12 @@ __device__ __forceinline__ Column load_column(const PointParams &P, const GeoCla
   return C;
 }
 
-template <int LP>
-__device__ __forceinline__ void load_pxy(const Task &T, const WavePlan &wp, const double *tab,
-                                         const Column &C, double px[LP + 1], double py[LP + 1]) {
+template <int LP, typename R>
+__device__ __forceinline__ void load_pxy(const Task &T, const WavePlan &wp, const R *tab,
+                                         const Column &C, R px[LP + 1], R py[LP + 1]) {
   const int np = wp.np;
-  const double *rx = tab + clampi(C.i - T.lo[0], 0, wp.ext - 1) * np;
-  const double *ry = tab + (wp.ext + clampi(C.j - T.lo[1], 0, wp.ext - 1)) * np;
+  const R *rx = tab + clampi(C.i - T.lo[0], 0, wp.ext - 1) * np;
+  const R *ry = tab + (wp.ext + clampi(C.j - T.lo[1], 0, wp.ext - 1)) * np;
 #pragma unroll
   for (int l = 0; l <= LP; l++) {
     px[l] = rx[l];
@@ -215,86 +221,73 @@ __device__ __forceinline__ void axis_offsets(const Task &T, const Layout &lay, i
   cw[2] = pm(T.center[2] - lay.shift_local[2], lay.npts_global[2]);
 }
 
-// Collocate the task's points; dst is the LDS window or the HBM level grid.
-template <int LP, typename Dst>
+// D[lz] = sum_ly py[ly] sum_lx cxyz[(lz, ly, lx)] px[lx] of one column.
+template <int LP, typename R>
+__device__ __forceinline__ void column_poly(const R *cx, const R *px, const R *py, R *D) {
+  int e = 0;
+#pragma unroll
+  for (int lz = 0; lz <= LP; lz++) {
+    R s = 0;
+#pragma unroll
+    for (int ly = 0; ly <= LP - lz; ly++) {
+      R t = 0;
+#pragma unroll
+      for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fmar(cx[e], px[lx], t);
+      s = fmar(t, py[ly], s);
+    }
+    D[lz] = s;
+  }
+}
+
+// Collocate the task's points straight into the HBM level grid.
+template <int LP, typename R>
 __device__ __forceinline__ void collocate_columns(const PointParams &P, const Task &T, const GeoClass &G,
-                                  const Layout &lay, const WavePlan &wp, const double *tab,
-                                  const double *cx, Dst *dst, int lane) {
+                                  const Layout &lay, const WavePlan &wp, const R *tab,
+                                  const R *cx, double *dst, int lane) {
   const int NP = wp.np;
-  const double *tabz = tab + 2 * wp.ext * NP;
+  const R *tabz = tab + 2 * wp.ext * NP;
   const int nxy = lay.npts_local[0] * lay.npts_local[1];
   const int ngz = lay.npts_global[2], nz = lay.npts_local[2];
   int cw[3];
   axis_offsets(T, lay, cw);
   for (int c0 = 0; c0 < G.ncol; c0 += 64) {
     const Column C = load_column(P, G, lay, cw, c0 + lane);
-    double px[LP + 1], py[LP + 1], D[LP + 1];
+    R px[LP + 1], py[LP + 1], D[LP + 1];
     load_pxy<LP>(T, wp, tab, C, px, py);
-    int e = 0;
-#pragma unroll
-    for (int lz = 0; lz <= LP; lz++) {
-      double s = 0.0;
-#pragma unroll
-      for (int ly = 0; ly <= LP - lz; ly++) {
-        double t = 0.0;
-#pragma unroll
-        for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fma(cx[e], px[lx], t);
-        s = fma(t, py[ly], s);
-      }
-      D[lz] = s;
-    }
+    column_poly<LP>(cx, px, py, D);
     const int kmin = P.geo_kmin[G.grp_off + (c0 >> 6)];
     const unsigned span = (unsigned)(2 - 2 * C.kl);  // 0 for an idle lane
     int gz = wrap(cw[2] + kmin, ngz);
 #pragma unroll 2
     for (int k = kmin; k <= 1 - kmin; k++) {
-      const double *pz = tabz + (k - T.lo[2]) * NP;
-      double v = 0.0;
+      const R *pz = tabz + (k - T.lo[2]) * NP;
+      R v = 0;
 #pragma unroll
-      for (int lz = 0; lz <= LP; lz++) v = fma(D[lz], pz[lz], v);
-      if (gz < nz && (unsigned)(k - C.kl) < span) atomicAdd(&dst[gz * nxy + C.base], v);
+      for (int lz = 0; lz <= LP; lz++) v = fmar(D[lz], pz[lz], v);
+      if (gz < nz && (unsigned)(k - C.kl) < span) atomicAdd(&dst[gz * nxy + C.base], (double)v);
       gz = (gz + 1 == ngz) ? 0 : gz + 1;
     }
-  }
-}
-
-// D[lz] = sum_ly py[ly] sum_lx cxyz[(lz, ly, lx)] px[lx] of one column.
-template <int LP>
-__device__ __forceinline__ void column_poly(const double *cx, const double *px, const double *py,
-                                            double *D) {
-  int e = 0;
-#pragma unroll
-  for (int lz = 0; lz <= LP; lz++) {
-    double s = 0.0;
-#pragma unroll
-    for (int ly = 0; ly <= LP - lz; ly++) {
-      double t = 0.0;
-#pragma unroll
-      for (int lx = 0; lx <= LP - lz - ly; lx++, e++) t = fma(cx[e], px[lx], t);
-      s = fma(t, py[ly], s);
-    }
-    D[lz] = s;
   }
 }
 
 // NCL groups of 64 columns at once (columns c0 + 64 u + lane, u < NCL): the
 // planes loop, its pz reads and its scalar bookkeeping serve NCL points per
 // lane and step.
-template <int LP, int NCL>
+template <int LP, int NCL, typename R>
 __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, const GeoClass &G,
                                            const Layout &lay, const WavePlan &wp,
-                                           const double *tab, const double *cx,
+                                           const R *tab, const R *cx,
                                            const TileItem &I, int gy0, int gz0, const int cw[3],
                                            int c0, int c_end, double *win, int lane) {
   const int NP = wp.np;
-  const double *tabz = tab + 2 * wp.ext * NP;
+  const R *tabz = tab + 2 * wp.ext * NP;
   const int nx = lay.npts_local[0];
   const int nxby = nx * wp.by;
   const int ngz = lay.npts_global[2];
   Column C[NCL];
   int lbase[NCL];
   unsigned span[NCL];
-  double D[NCL][LP + 1];
+  R D[NCL][LP + 1];
   int klmin = 1;
 #pragma unroll
   for (int u = 0; u < NCL; u++) {
@@ -308,7 +301,7 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
       else
         C[u].kl = 1;  // outside the tile: never for items built on the host
     }
-    double px[LP + 1], py[LP + 1];
+    R px[LP + 1], py[LP + 1];
     load_pxy<LP>(T, wp, tab, C[u], px, py);
     column_poly<LP>(cx, px, py, D[u]);
     span[u] = (unsigned)(2 - 2 * C[u].kl);  // 0 for an idle lane
@@ -319,15 +312,15 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
   int gz = wrap(cw[2] + ka, ngz);
 #pragma unroll 2
   for (int k = ka; k <= kb; k++) {
-    const double *pz = tabz + (k - T.lo[2]) * NP;
+    const R *pz = tabz + (k - T.lo[2]) * NP;
     const int tz = gz - gz0;
     if ((unsigned)tz < (unsigned)wp.bz) {
 #pragma unroll
       for (int u = 0; u < NCL; u++) {
-        double v = 0.0;
+        R v = 0;
 #pragma unroll
-        for (int lz = 0; lz <= LP; lz++) v = fma(D[u][lz], pz[lz], v);
-        if ((unsigned)(k - C[u].kl) < span[u]) atomicAdd(&win[tz * nxby + lbase[u]], v);
+        for (int lz = 0; lz <= LP; lz++) v = fmar(D[u][lz], pz[lz], v);
+        if ((unsigned)(k - C[u].kl) < span[u]) atomicAdd(&win[tz * nxby + lbase[u]], (double)v);
       }
     }
     gz = (gz + 1 == ngz) ? 0 : gz + 1;
@@ -336,41 +329,45 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
 
 // Collocate the rows j0..j1 and planes k0..k1 of a task into an LDS tile
 // win[(z - gz0) * nx * by + (y - gy0) * nx + x].
-template <int LP>
+template <int LP, typename R>
 __device__ __forceinline__ void tile_columns(const PointParams &P, const Task &T,
                                              const GeoClass &G, const Layout &lay,
-                                             const WavePlan &wp, const double *tab,
-                                             const double *cx, const TileItem &I, int gy0,
+                                             const WavePlan &wp, const R *tab,
+                                             const R *cx, const TileItem &I, int gy0,
                                              int gz0, double *win, int lane) {
+  // two columns per lane fit the registers for lp <= 4 (FP64) or lp <= 6 (FP32)
+  constexpr int L2 = sizeof(R) == 4 ? 6 : 4;
   int cw[3];
   axis_offsets(T, lay, cw);
   const int c_beg = P.geo_rows[G.row_off + (I.j0 - G.lo[1])];
   const int c_end = P.geo_rows[G.row_off + (I.j1 + 1 - G.lo[1])];
   for (int c0 = c_beg; c0 < c_end;) {
-    if (LP <= 4 && c_end - c0 > 64) {  // two columns per lane fit the registers for lp <= 4
-      tile_group<LP, (LP <= 4 ? 2 : 1)>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
-      c0 += (LP <= 4 ? 128 : 64);
+    if (LP <= L2 && c_end - c0 > 64) {
+      tile_group<LP, (LP <= L2 ? 2 : 1), R>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
+      c0 += (LP <= L2 ? 128 : 64);
     } else {
-      tile_group<LP, 1>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
+      tile_group<LP, 1, R>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
       c0 += 64;
     }
   }
 }
 
-// Integrate NCL groups of 64 columns at once into acc[(lz, ly, lx)].
-template <int LP, int NCL>
+// Integrate NCL groups of 64 columns at once into acc[(lz, ly, lx)]: the z
+// moments of a column in the evaluation type R, their outer product with px,
+// py accumulated in FP64.
+template <int LP, int NCL, typename R>
 __device__ __forceinline__ void integrate_group(const PointParams &P, const Task &T,
                                                 const GeoClass &G, const Layout &lay,
-                                                const WavePlan &wp, const double *tab,
-                                                const double *grid, const int cw[3], int c0,
+                                                const WavePlan &wp, const R *tab,
+                                                const R *grid, const int cw[3], int c0,
                                                 double *acc, int lane) {
   const int NP = wp.np;
-  const double *tabz = tab + 2 * wp.ext * NP;
+  const R *tabz = tab + 2 * wp.ext * NP;
   const int nxy = lay.npts_local[0] * lay.npts_local[1];
   const int ngz = lay.npts_global[2], nz = lay.npts_local[2];
   Column C[NCL];
   unsigned span[NCL];
-  double M[NCL][LP + 1];
+  R M[NCL][LP + 1];
   int kmin = 1;
 #pragma unroll
   for (int u = 0; u < NCL; u++) {
@@ -378,63 +375,63 @@ __device__ __forceinline__ void integrate_group(const PointParams &P, const Task
     span[u] = (unsigned)(2 - 2 * C[u].kl);
     kmin = min(kmin, P.geo_kmin[G.grp_off + (c0 >> 6) + u]);
 #pragma unroll
-    for (int l = 0; l <= LP; l++) M[u][l] = 0.0;
+    for (int l = 0; l <= LP; l++) M[u][l] = 0;
   }
   int gz = wrap(cw[2] + kmin, ngz);
 #pragma unroll 4  // several grid loads in flight per wave
   for (int k = kmin; k <= 1 - kmin; k++) {
-    const double *pz = tabz + (k - T.lo[2]) * NP;
-    double g[NCL];
+    const R *pz = tabz + (k - T.lo[2]) * NP;
+    R g[NCL];
 #pragma unroll
     for (int u = 0; u < NCL; u++) {
-      g[u] = 0.0;
+      g[u] = 0;
       if (gz < nz && (unsigned)(k - C[u].kl) < span[u]) g[u] = grid[gz * nxy + C[u].base];
     }
 #pragma unroll
     for (int u = 0; u < NCL; u++)
 #pragma unroll
-      for (int l = 0; l <= LP; l++) M[u][l] = fma(g[u], pz[l], M[u][l]);
+      for (int l = 0; l <= LP; l++) M[u][l] = fmar(g[u], pz[l], M[u][l]);
     gz = (gz + 1 == ngz) ? 0 : gz + 1;
   }
 #pragma unroll
   for (int u = 0; u < NCL; u++) {
-    double px[LP + 1], py[LP + 1];
+    R px[LP + 1], py[LP + 1];
     load_pxy<LP>(T, wp, tab, C[u], px, py);
     int e = 0;
 #pragma unroll
     for (int lz = 0; lz <= LP; lz++)
 #pragma unroll
       for (int ly = 0; ly <= LP - lz; ly++) {
-        const double yz = py[ly] * M[u][lz];
+        const double yz = (double)py[ly] * (double)M[u][lz];
 #pragma unroll
-        for (int lx = 0; lx <= LP - lz - ly; lx++, e++) acc[e] = fma(px[lx], yz, acc[e]);
+        for (int lx = 0; lx <= LP - lz - ly; lx++, e++) acc[e] = fma((double)px[lx], yz, acc[e]);
       }
   }
 }
 
 // Integrate the task's points into lane-private accumulators acc[(lz, ly, lx)].
-template <int LP>
+template <int LP, typename R>
 __device__ __forceinline__ void integrate_columns(const PointParams &P, const Task &T,
                                                   const GeoClass &G, const Layout &lay,
-                                                  const WavePlan &wp, const double *tab,
-                                                  const double *grid, double *acc, int lane) {
+                                                  const WavePlan &wp, const R *tab,
+                                                  const R *grid, double *acc, int lane) {
   int cw[3];
   axis_offsets(T, lay, cw);
   // one column per lane: two (integrate_group<LP, 2>) cost occupancy (156
   // VGPRs) and measured slower on h2o64
   for (int c0 = 0; c0 < G.ncol; c0 += 64)
-    integrate_group<LP, 1>(P, T, G, lay, wp, tab, grid, cw, c0, acc, lane);
+    integrate_group<LP, 1, R>(P, T, G, lay, wp, tab, grid, cw, c0, acc, lane);
 }
 
-template <int LP>
+template <int LP, typename R>
 __device__ __forceinline__ void integrate_task(const PointParams &P, const Task &T, const GeoClass &G,
-                               const Layout &lay, const WavePlan &wp, const double *tab,
-                               const double *grid, double *out, int lane) {
+                               const Layout &lay, const WavePlan &wp, const R *tab,
+                               const R *grid, double *out, int lane) {
   constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
   double acc[NC];
 #pragma unroll
   for (int e = 0; e < NC; e++) acc[e] = 0.0;
-  integrate_columns<LP>(P, T, G, lay, wp, tab, grid, acc, lane);
+  integrate_columns<LP, R>(P, T, G, lay, wp, tab, grid, acc, lane);
 #pragma unroll
   for (int e = 0; e < NC; e++) {
     const double s = wave_total(acc[e]);
@@ -680,8 +677,19 @@ __global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__
   }
 }
 
+// Packed FP64 cxyz of a task -> the wave's LDS, in the evaluation type R.
+template <typename R>
+__device__ __forceinline__ const R *stage_cxyz(const PointParams &P, long long task, int lp,
+                                               const WaveLds &W, int lane) {
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  const double *src = P.cxyz_all + task * P.cxyz_stride;
+  R *cx = reinterpret_cast<R *>(W.cxyz);
+  for (int e = lane; e < nc; e += 64) cx[e] = (R)src[e];
+  return cx;
+}
+
 // Collocation straight into HBM, one task per wave.
-template <int LPB>
+template <int LPB, typename R>
 __global__ __launch_bounds__(256) void collocate_wave_kernel(const Task *__restrict__ tasks,
                                                              const Level *__restrict__ levels,
                                                              PointParams P) {
@@ -698,27 +706,26 @@ __global__ __launch_bounds__(256) void collocate_wave_kernel(const Task *__restr
   const Layout &lay = L.lay;
   const WaveLds W = wave_lds(lds, wp, wave);
   const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
-  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
-  const double *src = P.cxyz_all + task * P.cxyz_stride;
-  for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
-  wave_tables(T, lay, lp, wp, W.u, lane);  // ends with wave_sync
-  const double *cx = W.cxyz;
+  const R *cx = stage_cxyz<R>(P, task, lp, W, lane);
+  R *tab = reinterpret_cast<R *>(W.u);
+  wave_tables(T, lay, lp, wp, tab, lane);  // ends with wave_sync
   double *dst = L.grid;
   switch (lp) {
-    case 0: collocate_columns<0>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-    case 1: collocate_columns<1>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-    case 2: collocate_columns<2>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-    case 3: collocate_columns<3>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-    case 4: collocate_columns<4>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-    case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
-    case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, cx, dst, lane); break;
+    case 0: collocate_columns<0>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 1: collocate_columns<1>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 2: collocate_columns<2>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 3: collocate_columns<3>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 4: collocate_columns<4>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, cx, dst, lane); break;
     default: break;  // the host routes larger lp to the per-task kernels
   }
 }
 
 // Collocation through LDS tiles: workgroup = unit P.unit_base + blockIdx.x,
-// the items U.ibeg + U.first + m * U.stride < U.iend of one tile.
-template <int LPB>
+// the items U.ibeg + U.first + m * U.stride < U.iend of one tile.  The tile
+// accumulates in FP64 whatever the evaluation type.
+template <int LPB, typename R>
 __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
     const Task *__restrict__ tasks, const Level *__restrict__ levels,
     const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
@@ -733,25 +740,23 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
   for (int e = threadIdx.x; e < wp.window; e += blockDim.x) lds[e] = 0.0;
   __syncthreads();
   const WaveLds W = wave_lds(lds, wp, wave);
+  R *tab = reinterpret_cast<R *>(W.u);
   for (long long it = U.ibeg + U.first + (long long)wave * U.stride; it < U.iend;
        it += (long long)nw * U.stride) {
     const TileItem I = items[it];
     const Task &T = tasks[I.task];
     const GeoClass &G = P.geo[T.geo];
     const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
-    const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
-    const double *src = P.cxyz_all + (long long)I.task * P.cxyz_stride;
-    for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
-    wave_tables(T, lay, lp, wp, W.u, lane);  // ends with wave_sync
-    const double *cx = W.cxyz;
+    const R *cx = stage_cxyz<R>(P, I.task, lp, W, lane);
+    wave_tables(T, lay, lp, wp, tab, lane);  // ends with wave_sync
     switch (lp) {
-      case 0: tile_columns<0>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 1: tile_columns<1>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 2: tile_columns<2>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 3: tile_columns<3>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 4: tile_columns<4>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 5: if (LPB >= 5) tile_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 6: if (LPB >= 6) tile_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 0: tile_columns<0>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 1: tile_columns<1>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 2: tile_columns<2>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 3: tile_columns<3>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 4: tile_columns<4>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 5: if (LPB >= 5) tile_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 6: if (LPB >= 6) tile_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
       default: break;  // the host routes larger lp to the per-task kernels
     }
     wave_sync();
@@ -767,7 +772,8 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
   }
 }
 
-template <int LPB>
+// R = float reads the FP32 copy of the potential (Level::gridf).
+template <int LPB, typename R>
 __global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel(
     const Task *__restrict__ tasks, const Level *__restrict__ levels, PointParams P) {
   extern __shared__ double lds[];
@@ -783,17 +789,22 @@ __global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel
   const Layout &lay = L.lay;
   const WaveLds W = wave_lds(lds, wp, wave);
   const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
-  wave_tables(T, lay, lp, wp, W.u, lane);
+  R *tab = reinterpret_cast<R *>(W.u);
+  wave_tables(T, lay, lp, wp, tab, lane);
   double *out = P.cxyz_all + task * P.cxyz_stride;
-  const double *grid = L.grid;
+  const R *grid;
+  if constexpr (sizeof(R) == 4)
+    grid = L.gridf;
+  else
+    grid = L.grid;
   switch (lp) {
-    case 0: integrate_task<0>(P, T, G, lay, wp, W.u, grid, out, lane); break;
-    case 1: integrate_task<1>(P, T, G, lay, wp, W.u, grid, out, lane); break;
-    case 2: integrate_task<2>(P, T, G, lay, wp, W.u, grid, out, lane); break;
-    case 3: integrate_task<3>(P, T, G, lay, wp, W.u, grid, out, lane); break;
-    case 4: integrate_task<4>(P, T, G, lay, wp, W.u, grid, out, lane); break;
-    case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, W.u, grid, out, lane); break;
-    case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, W.u, grid, out, lane); break;
+    case 0: integrate_task<0>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 1: integrate_task<1>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 2: integrate_task<2>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 3: integrate_task<3>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 4: integrate_task<4>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
     default: break;  // the host routes larger lp to the per-task kernels
   }
 }
@@ -852,11 +863,25 @@ hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s) {
   return launch_wave(cxyz_wave_kernel, P, wave_blocks(P), s, P.tasks, P.groups);
 }
 
+template <typename R>
+static hipError_t collocate_wave_r(const PointParams &P, int lpb, hipStream_t s) {
+  if (lpb == 4) return launch_wave(collocate_wave_kernel<4, R>, P, wave_blocks(P), s, P.tasks, P.levels);
+  if (lpb == 6) return launch_wave(collocate_wave_kernel<6, R>, P, wave_blocks(P), s, P.tasks, P.levels);
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
-  if (lpb == 4) return launch_wave(collocate_wave_kernel<4>, P, wave_blocks(P), s, P.tasks, P.levels);
-  if (lpb == 6) return launch_wave(collocate_wave_kernel<6>, P, wave_blocks(P), s, P.tasks, P.levels);
+  return P.fp32 ? collocate_wave_r<float>(P, lpb, s) : collocate_wave_r<double>(P, lpb, s);
+}
+
+template <typename R>
+static hipError_t collocate_tiles_r(const PointParams &P, int lpb, int nunits, hipStream_t s) {
+  if (lpb == 4)
+    return launch_wave(collocate_tile_kernel<4, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
+  if (lpb == 6)
+    return launch_wave(collocate_tile_kernel<6, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
   return hipErrorInvalidValue;
 }
 
@@ -865,19 +890,21 @@ hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hip
   if (nunits <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0)
     return hipErrorInvalidValue;
-  if (lpb == 4)
-    return launch_wave(collocate_tile_kernel<4>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
-  if (lpb == 6)
-    return launch_wave(collocate_tile_kernel<6>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
+  return P.fp32 ? collocate_tiles_r<float>(P, lpb, nunits, s)
+                : collocate_tiles_r<double>(P, lpb, nunits, s);
+}
+
+template <typename R>
+static hipError_t integrate_wave_r(const PointParams &P, int lpb, hipStream_t s) {
+  if (lpb == 4) return launch_wave(integrate_wave_kernel<4, R>, P, wave_blocks(P), s, P.tasks, P.levels);
+  if (lpb == 6) return launch_wave(integrate_wave_kernel<6, R>, P, wave_blocks(P), s, P.tasks, P.levels);
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
-  if (lpb == 4) return launch_wave(integrate_wave_kernel<4>, P, wave_blocks(P), s, P.tasks, P.levels);
-  if (lpb == 6) return launch_wave(integrate_wave_kernel<6>, P, wave_blocks(P), s, P.tasks, P.levels);
-  return hipErrorInvalidValue;
+  return P.fp32 ? integrate_wave_r<float>(P, lpb, s) : integrate_wave_r<double>(P, lpb, s);
 }
 
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s) {

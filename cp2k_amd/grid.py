@@ -43,10 +43,10 @@ FP32_MIXED = 1
 
 KERNEL_COLLOCATE, KERNEL_INTEGRATE, KERNEL_DECONTRACT, KERNEL_CONTRACT, \
     KERNEL_FILL, KERNEL_REDUCE, KERNEL_CXYZ, KERNEL_TILES, KERNEL_DIRECT, \
-    KERNEL_INTEGRATE_POINTS, KERNEL_FINISH, KERNEL_PER_TASK = range(12)
+    KERNEL_INTEGRATE_POINTS, KERNEL_FINISH, KERNEL_PER_TASK, KERNEL_TO_FLOAT = range(13)
 KERNEL_NAMES = ["collocate", "integrate", "decontract", "contract", "fill", "reduce",
                 "cxyz", "collocate_tiles", "collocate_direct", "integrate_points",
-                "finish", "per_task"]
+                "finish", "per_task", "to_float"]
 
 
 class GridBasisSet(C.Structure):

@@ -54,9 +54,14 @@ void grid_gpu_integrate_task_list(const grid_gpu_task_list *ptr,
 
 /* Arithmetic of the per-point stage.  FP64: everything in double (matches
  * the reference to summation order).  FP32_MIXED: the Gaussian x polynomial
- * evaluation at each grid point runs in float; coefficient transforms,
- * factor tables, index math and every accumulation stay double.  Default
- * FP64, or the value of environment variable GRID_MI355X_PRECISION (0/1). */
+ * evaluation at each grid point runs in float (orthorhombic tasks with
+ * la_max + lb_max <= 6 after the l shifts of the call: factor tables rounded
+ * to float, per-column polynomials, per-point sums; integrate reads the
+ * potential rounded to float; general cells: the per-point exponential in
+ * float); coefficient transforms, the factor tables' computation, index math,
+ * grid accumulation, hab and the force/virial reductions stay double.  Tasks
+ * with larger l stay FP64.  Default FP64, or the value of environment
+ * variable GRID_MI355X_PRECISION (0/1). */
 enum grid_mi355x_precision {
   GRID_MI355X_FP64 = 0,
   GRID_MI355X_FP32_MIXED = 1,
@@ -98,7 +103,8 @@ void grid_mi355x_synchronize(void);
  * call: ids 9, 10, 11 of it), 2 decontract pab, 3 contract hab, 4 zero/fill,
  * 5 force/virial reduction, 6 cab -> cxyz pass, 7 collocation through LDS
  * tiles, 8 collocation straight into HBM, 9 integrate points, 10 cxyz ->
- * hab/forces pass, 11 per-task kernels (general cells, lp > 6). */
+ * hab/forces pass, 11 per-task kernels (general cells, lp > 6), 12 rounding
+ * of the potential to float (FP32-mixed integrate). */
 void grid_mi355x_set_timing(const bool enable);
 void grid_mi355x_reset_timing(void);
 /* Returns total milliseconds and number of launches of kernel id. */

@@ -136,6 +136,29 @@ def test_device_api_matches_host_api():
     assert replay.rel_err(v_d.numpy(), v) < 1e-13
 
 
+def test_device_calls_in_flight():
+    """Device-API calls return before their kernels run: back-to-back
+    collocate/integrate pairs (more than the library's eight per-call level
+    tables) without a synchronisation must each use their own buffers."""
+    args, pab, V = synth.config("h2o2", seed=9, n=32)
+    gt = grid.GridTaskList(args)
+    want_rho = gt.collocate(100, pab)
+    want_hab, _, _ = gt.integrate(V)
+    pab_d = grid.DeviceArray.from_numpy(pab)
+    rho_d = [grid.DeviceArray(s) for s in args.grid_shapes()]
+    V_d = [grid.DeviceArray.from_numpy(v) for v in V]
+    hab_d = grid.DeviceArray(args.buffer_size)
+    for _ in range(12):
+        gt.collocate_dev(100, pab_d, rho_d)
+        gt.integrate_dev(V_d, hab_d)
+    grid.synchronize()
+    for v_d, v in zip(V_d, V):
+        assert np.array_equal(v_d.numpy(), v)
+    for r_d, r in zip(rho_d, want_rho):
+        assert replay.rel_err(r_d.numpy(), r) < 1e-14
+    assert replay.rel_err(hab_d.numpy(), want_hab) < 1e-14
+
+
 def test_task_points_match_oracle():
     args, pab, V = synth.config("h2o2", seed=4, n=32)
     assert grid.GridTaskList(args).task_points() == O.TaskList(args).task_points()
