@@ -16,7 +16,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libcp2k_grid_mi355x.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = [CSRC / "grid_api.cpp", CSRC / "grid_kernels.hip", CSRC / "grid_wave.hip"]
+SOURCES = [CSRC / "grid_api.cpp", CSRC / "grid_kernels.hip", CSRC / "grid_wave.hip",
+           CSRC / "grid_general.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-munsafe-fp-atomics",

@@ -48,6 +48,8 @@ hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
+hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
+hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s);
 }  // namespace gm
 
 using namespace gm;
@@ -82,7 +84,7 @@ static inline int pmod(int a, int m) { return ((a % m) + m) % m; }
 // Library state (src/grid/common/grid_library.c)
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kNumKernelIds = 13;
+constexpr int kNumKernelIds = 14;
 constexpr int kCounterLp = 20;
 
 struct PendingTiming {
@@ -232,6 +234,60 @@ bool func_ldiffs(int func, int d[4]) {
 }
 
 // process_get_ldiffs, src/grid/common/grid_process_vab.h:222-251.
+// Packed index of (a, b, c), a + b + c <= lp: a fastest, c slowest (the
+// layout of the device's packed cxyz and cijk).
+int packed_idx(int a, int b, int c, int lp) {
+  int off = 0;
+  for (int z = 0; z < c; z++) off += (lp - z + 1) * (lp - z + 2) / 2;
+  for (int y = 0; y < b; y++) off += lp - c - y + 1;
+  return off + a;
+}
+
+double factorial(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; i++) f *= i;
+  return f;
+}
+
+// Appends M[o][i] and then M^T for the change of variables of a general
+// cell: cijk[o] = sum_i M[o][i] cxyz[i] with (x - xp)^lx (y - yp)^ly
+// (z - zp)^lz expanded in the lattice offsets (di, dj, dk) through
+// r = di dh[0] + dj dh[1] + dk dh[2] (the multinomial expansion of
+// src/grid/ref/grid_ref_collint.h:706-761).
+void append_general_matrix(const double dh[3][3], int lp, std::vector<double> &out) {
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  std::vector<double> M((size_t)nc * nc, 0.0);
+  // (x - xp)^lx = (di dh00 + dj dh10 + dk dh20)^lx = sum lx! / (a! b! c!) ...
+  for (int lz = 0; lz <= lp; lz++)
+    for (int ly = 0; ly <= lp - lz; ly++)
+      for (int lx = 0; lx <= lp - lz - ly; lx++) {
+        const int in = packed_idx(lx, ly, lz, lp);
+        const int L[3] = {lx, ly, lz};
+        // split each Cartesian power over the three lattice directions
+        std::vector<std::array<int, 3>> parts[3];
+        for (int d = 0; d < 3; d++)
+          for (int a = 0; a <= L[d]; a++)
+            for (int b = 0; b <= L[d] - a; b++) parts[d].push_back({a, b, L[d] - a - b});
+        for (const auto &px : parts[0])
+          for (const auto &py : parts[1])
+            for (const auto &pz : parts[2]) {
+              const std::array<int, 3> *P3[3] = {&px, &py, &pz};
+              double m = 1.0;
+              for (int d = 0; d < 3; d++) {
+                const auto &pd = *P3[d];
+                m *= factorial(L[d]) / (factorial(pd[0]) * factorial(pd[1]) * factorial(pd[2]));
+                for (int e = 0; e < 3; e++) m *= std::pow(dh[e][d], pd[e]);
+              }
+              const int il = px[0] + py[0] + pz[0], jl = px[1] + py[1] + pz[1],
+                        kl = px[2] + py[2] + pz[2];
+              M[(size_t)packed_idx(il, jl, kl, lp) * nc + in] += m;
+            }
+      }
+  out.insert(out.end(), M.begin(), M.end());
+  for (int i = 0; i < nc; i++)
+    for (int o = 0; o < nc; o++) out.push_back(M[(size_t)o * nc + i]);
+}
+
 void integrate_ldiffs(bool forces, bool virial, bool tau, int d[4]) {
   d[0] = d[1] = d[2] = d[3] = 0;
   if (forces || virial) { d[0] += 1; d[1] -= 1; d[3] -= 1; }
@@ -293,6 +349,7 @@ struct TaskListImpl {
   DevBuf d_geo, d_geo_cols, d_geo_kmin, d_cxyz;  // geometry classes, per-task cxyz
   DevBuf d_geo_rows, d_items;                      // row starts, collocation tile items
   DevBuf d_gridf;                                  // FP32-mixed: potential rounded to FP32
+  DevBuf d_gmat, d_gmat_off;                       // general cells: cxyz -> cijk matrices
   std::vector<TileInfo> tiles;
   std::map<int, TilePlan> tile_plans;              // by lp shift
   // Per-call level tables.  A call returns once its work is enqueued, so the
@@ -317,6 +374,7 @@ struct TaskListImpl {
   long n_ortho = 0, n_general = 0, n_skip = 0;
   long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
   std::vector<OrthoLevel> olev;   // per level
+  std::vector<OrthoLevel> glev;   // per level: general tasks (begin, end, lp_start)
   std::vector<signed char> lp0s;  // lp0 per sorted task
   std::vector<int> highl_before;  // sorted tasks before s with la_max or lb_max > 2
   int ngeo = 0;
@@ -332,7 +390,7 @@ struct TaskListImpl {
     DevBuf *bufs[] = {&d_tasks, &d_groups, &d_sphi, &d_fptr, &d_fent,
                       &d_cart, &d_hab_cart, &d_task_out, &pab_dev, &hab_dev, &forces_dev,
                       &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz, &d_geo_rows, &d_items,
-                      &d_gridf};
+                      &d_gridf, &d_gmat, &d_gmat_off};
     for (auto &kv : tile_plans) kv.second.d.release();
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
@@ -845,12 +903,37 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
       a = b;
     }
   }
-  for (auto &o : tl->olev)
-    for (int v = 0; v < kLp0Slots; v++) {
-      int s = o.begin;
-      while (s < o.end && tl->lp0s[s] < v) s++;
-      o.lp_start[v] = s;
-    }
+  // --- per-level ranges of the general tasks (sorted by level, then lp0) ---
+  tl->glev.assign(nlevels, OrthoLevel{});
+  for (int s = 0; s < ntasks; s++) {
+    const Task &T = tasks[s];
+    if ((T.flags & (kTaskOrtho | kTaskSkip)) != 0) continue;
+    OrthoLevel &g = tl->glev[T.level];
+    if (g.end == 0) g.begin = s;
+    g.end = s + 1;
+  }
+  for (auto *lv : {&tl->olev, &tl->glev})
+    for (auto &o : *lv)
+      for (int v = 0; v < kLp0Slots; v++) {
+        int s = o.begin;
+        while (s < o.end && tl->lp0s[s] < v) s++;
+        o.lp_start[v] = s;
+      }
+
+  // --- general cells: per level and lp <= 6 the change of variables cxyz
+  //     -> cijk (src/grid/ref/grid_ref_collint.h:697-762) as a matrix, and
+  //     its transpose ---
+  if (tl->has_general) {
+    std::vector<double> gmat;
+    std::vector<int> goff((size_t)nlevels * 8, 0);
+    for (int l = 0; l < nlevels; l++)
+      for (int lp = 0; lp <= 6; lp++) {
+        goff[(size_t)l * 8 + lp] = (int)gmat.size();
+        append_general_matrix(tl->layouts[l].dh, lp, gmat);
+      }
+    upload(tl->d_gmat, gmat);
+    upload(tl->d_gmat_off, goff);
+  }
 
   // --- force CSR: atom -> (2*task + side) ---
   std::vector<int> fptr(natoms + 1, 0), fent(2 * (size_t)ntasks);
@@ -1134,6 +1217,74 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   }
 }
 
+// LDS plan of the general point kernels: packed cxyz and dense cijk per
+// wave, then the collocation staging: 64 x 9 values and 64 x 9 indices.
+WavePlan general_plan(int lp) {
+  WavePlan w{};
+  w.np = lp + 1;
+  w.o_cxyz = 0;
+  w.o_u = w.np * w.np * w.np;
+  w.wave_bytes = (16 * w.o_u + 8 * 64 * 9 + 4 * 64 * 9 + 15) & ~15;
+  w.nwaves = 4;
+  w.bytes = w.nwaves * w.wave_bytes;
+  return w;
+}
+
+// General tasks, level by level: wave-per-task kernels for lp <= 6 (cab ->
+// cxyz pass, point kernel; integrate: point kernel, cxyz -> hab pass), the
+// per-task kernels for larger lp.
+void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+  const int sh = P.ld[0] + P.ld[2];
+  for (int l = 0; l < tl->nlevels; l++) {
+    const OrthoLevel &g = tl->glev[l];
+    if (g.end <= g.begin) continue;
+    auto first = [&](int v) { return g.lp_start[std::min(std::max(v, 0), kLp0Slots - 1)]; };
+    int c1 = first(5 - sh), c2 = first(7 - sh);  // lp <= 4 | lp 5..6 | larger
+    if (!wave_kernels()) c1 = c2 = g.begin;
+    if (c2 > g.begin) {
+      const int lp_all = std::min(6, tl->lp0s[c2 - 1] + sh);
+      PointParams Q = P;
+      Q.task_base = g.begin;
+      Q.ntasks = c2 - g.begin;
+      Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanSetup);
+      if (mode == kCollocate) {
+        TimedLaunch tm(6, s);
+        const bool ab = P.func == 100 && P.ld[0] == 0 && P.ld[1] == 0 && P.ld[2] == 0 &&
+                        P.ld[3] == 0;
+        Q.lane_cxyz = ab ? 1 : 0;
+        if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
+        if (!ab || tl->highl_before[c2] > tl->highl_before[g.begin])
+          HIP_CHECK(launch_cxyz_wave(Q, s));
+      }
+      const int cuts[2][2] = {{g.begin, c1}, {c1, c2}};
+      for (int r = 0; r < 2; r++) {
+        const int a = cuts[r][0], b = cuts[r][1];
+        if (b <= a) continue;
+        const int lpb = r == 0 ? 4 : 6;
+        PointParams R = P;
+        R.task_base = a;
+        R.ntasks = b - a;
+        R.wp = general_plan(std::min(lpb, tl->lp0s[b - 1] + sh));
+        TimedLaunch tm(13, s);
+        HIP_CHECK(mode == kIntegrate ? launch_general_integrate(R, lpb, s)
+                                     : launch_general_collocate(R, lpb, s));
+      }
+      if (mode == kIntegrate) {
+        TimedLaunch tm(10, s);
+        Q.lane_cxyz = 0;
+        HIP_CHECK(launch_finish_wave(Q, s));
+      }
+    }
+    if (g.end > c2) {
+      PointParams Q = P;
+      Q.task_base = c2;
+      Q.ntasks = g.end - c2;
+      TimedLaunch tm(11, s);
+      HIP_CHECK(mode == kIntegrate ? launch_integrate(Q, false, s) : launch_collocate(Q, false, s));
+    }
+  }
+}
+
 // Launches the orthorhombic and the general task ranges (tasks are sorted
 // by class: orthorhombic, general, skipped).
 void launch_ranges(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
@@ -1150,11 +1301,12 @@ void launch_ranges(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   } else if (tl->n_ortho > 0) {
     launch_ortho(tl, P, s, mode);
   }
-  if (tl->n_general > 0) {
+  if (tl->n_general > 0 && mode != kCount) {
+    launch_general(tl, P, s, mode);
+  } else if (tl->n_general > 0) {
     P.task_base = (int)tl->n_ortho;
     P.ntasks = (int)tl->n_general;
-    TimedLaunch tm(mode == kCount ? -1 : 11, s);
-    HIP_CHECK(mode == kIntegrate ? launch_integrate(P, false, s) : launch_collocate(P, false, s));
+    HIP_CHECK(launch_collocate(P, false, s));
   }
 }
 
@@ -1166,6 +1318,8 @@ void set_wave_buffers(TaskListImpl *tl, PointParams &P) {
   P.geo_rows = (const int *)tl->d_geo_rows.p;
   P.items = (const TileItem *)tl->d_items.p;
   P.cxyz_stride = cxyz_stride(tl, P.ld[0] + P.ld[2]);
+  P.gmat = (const double *)tl->d_gmat.p;
+  P.gmat_off = (const int *)tl->d_gmat_off.p;
   P.cxyz_all = (double *)tl->d_cxyz.get(sizeof(double) * (size_t)P.cxyz_stride *
                                         (size_t)std::max(tl->ntasks, 1));
 }

@@ -1,0 +1,381 @@
+// Wave-per-task kernels for general tasks: non-orthorhombic cells or tasks
+// with a border mask (gfx950, wave64).
+//
+// A general task covers the lattice index box lo..hi (from the 27 corners of
+// the cube around its sphere) and, in each row (j, k) of the box whose grid
+// indices lie inside the border-masked local grid, the points i of the
+// quadratic bound of src/grid/ref/grid_ref_collint.h:446-464.  One wave
+// takes one task; each lane walks whole rows.  Along a row r(i) = r0 + i h0
+// (h0 = dh[0]), so the Gaussian follows the recurrence
+//   g(i + 1) = g(i) q(i),   q(i + 1) = q(i) exp(-2 zetp |h0|^2),
+// restarted from exp() every kRestart points; a point costs two multiplies
+// instead of the reference's three-table product (:358-367).
+//
+// Collocate: cxyz (from the cab -> cxyz pass) -> cijk, the polynomial in the
+// lattice offsets (di, dj, dk) (:697-762); per row ci[il] = sum cijk dj^jl
+// dk^kl, per point a Horner sum in di times the Gaussian; 64 rows x 8 points
+// are staged in LDS and added to the level grid in HBM point-major.  Integrate: per row the moments sum_i V g di^il, folded into
+// lane-private cijk accumulators, one cross-lane reduction per task, cijk ->
+// cxyz, then the shared cxyz -> hab/forces pass.
+//
+// Compiled with -ffp-contract=off; fused multiply-adds are explicit fma().
+
+#include <hip/hip_runtime.h>
+
+#include "grid_device.h"
+#include "grid_internal.h"
+
+namespace gm {
+namespace {
+
+constexpr int kRestart = 32;  // points between exact exp() evaluations of the recurrence
+
+struct GenLds {
+  double *cxyz;  // dense [np][np][np]
+  double *cijk;  // dense [np][np][np], then (collocate) the staging of GenStage
+};
+
+__device__ __forceinline__ GenLds gen_lds(double *lds, const WavePlan &wp, int wave) {
+  double *b = reinterpret_cast<double *>(reinterpret_cast<char *>(lds) +
+                                         (size_t)wave * wp.wave_bytes);
+  return GenLds{b + wp.o_cxyz, b + wp.o_u};
+}
+
+// Row (j, k) of the task's box: grid row base and the point range i0..i1;
+// false for a row outside the border-masked grid or outside the sphere.
+__device__ __forceinline__ bool gen_row(const Task &T, const Layout &lay, int j, int k, int &i0,
+                                        int &i1, long long &base) {
+  const int gk = wrap(k - lay.shift_local[2], lay.npts_global[2]);
+  if (gk < T.blo[2] || gk > T.bhi[2]) return false;
+  const int gj = wrap(j - lay.shift_local[1], lay.npts_global[1]);
+  if (gj < T.blo[1] || gj > T.bhi[1]) return false;
+  if (!general_row(lay.dh, T.gp, j, k, T.radius, &i0, &i1)) return false;
+  // the sphere lies inside the index box; the clamp guards rounding
+  i0 = max(i0, T.lo[0]);
+  i1 = min(i1, T.hi[0]);
+  if (i0 > i1) return false;
+  base = ((long long)gk * lay.npts_local[1] + gj) * lay.npts_local[0];
+  return true;
+}
+
+// Gaussian and its ratio at point i of row (j, k).
+__device__ __forceinline__ void gen_gauss(const Task &T, const Layout &lay, int i, int j, int k,
+                                          double &g, double &q) {
+  const double di = i - T.gp[0], dj = j - T.gp[1], dk = k - T.gp[2];
+  double r2 = 0.0, rh = 0.0, hh = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double r = di * lay.dh[0][d] + dj * lay.dh[1][d] + dk * lay.dh[2][d];
+    r2 += r * r;
+    rh += r * lay.dh[0][d];
+    hh += lay.dh[0][d] * lay.dh[0][d];
+  }
+  g = exp(-T.zetp * r2);
+  q = exp(-T.zetp * (2.0 * rh + hh));
+}
+
+// (a, b, c) of packed index o (inverse of packed_index).
+__device__ __forceinline__ void packed_unpack(int o, int lp, int &a, int &b, int &c) {
+  c = 0;
+  for (int n = (lp + 1) * (lp + 2) / 2; o >= n; n = (lp - c + 1) * (lp - c + 2) / 2) {
+    o -= n;
+    c++;
+  }
+  b = 0;
+  for (int n = lp - c + 1; o >= n; n = lp - c - b + 1) {
+    o -= n;
+    b++;
+  }
+  a = o;
+}
+
+// cxyz (packed, from HBM) -> dense cijk in the wave's LDS through the
+// level's matrix: cijk[o] = sum_i M^T[i][o] cxyz[i], one output per lane.
+__device__ void gen_cijk(const PointParams &P, const Task &T, long long task, int lp,
+                         const GenLds &W, int NP, int lane) {
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  const double *src = P.cxyz_all + task * P.cxyz_stride;
+  for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
+  wave_sync();
+  const double *MT = P.gmat + P.gmat_off[T.level * 8 + lp] + nc * nc;
+  for (int o = lane; o < nc; o += 64) {
+    double s = 0.0;
+    for (int i = 0; i < nc; i++) s = fma(MT[i * nc + o], W.cxyz[i], s);
+    int il, jl, kl;
+    packed_unpack(o, lp, il, jl, kl);
+    W.cijk[(kl * NP + jl) * NP + il] = s;
+  }
+  wave_sync();
+}
+
+// Collocation staging of one wave: 64 rows x kChunk points, stored lane
+// (row) major and read back point major, so that each atomic instruction
+// covers kChunk consecutive points of 64 / kChunk rows instead of one point
+// of 64 rows.
+constexpr int kChunk = 8;
+struct GenStage {
+  double *val;  // [64][kChunk + 1]
+  int *idx;     // [64][kChunk + 1] grid index, -1 for no point
+};
+
+__device__ __forceinline__ GenStage gen_stage(const GenLds &W, int NP) {
+  GenStage g;
+  g.val = W.cijk + NP * NP * NP;
+  g.idx = reinterpret_cast<int *>(g.val + 64 * (kChunk + 1));
+  return g;
+}
+
+template <int LP>
+__device__ __forceinline__ void gen_collocate_rows(const Task &T, const Layout &lay,
+                                                   const double *cijk, int NP,
+                                                   const GenStage &S, double *grid, int lane) {
+  const int nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
+  const int ngx = lay.npts_global[0];
+  double hh = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
+  const double r2step = exp(-2.0 * T.zetp * hh);
+  for (int r0 = 0; r0 < nj * nk; r0 += 64) {
+    const int r = r0 + lane;
+    const int j = T.lo[1] + r % nj, k = T.lo[2] + r / nj;
+    int i0 = 0, i1 = -1;
+    long long base = 0;
+    if (r < nj * nk && !gen_row(T, lay, j, k, i0, i1, base)) i1 = i0 - 1;
+    if (r >= nj * nk) i1 = i0 - 1;
+    // ci[il] = sum_{jl, kl} cijk[kl][jl][il] dj^jl dk^kl
+    const double dj = j - T.gp[1], dk = k - T.gp[2];
+    double ci[LP + 1];
+#pragma unroll
+    for (int il = 0; il <= LP; il++) ci[il] = 0.0;
+    double dkp = 1.0;
+#pragma unroll
+    for (int kl = 0; kl <= LP; kl++) {
+      double w = dkp;
+#pragma unroll
+      for (int jl = 0; jl <= LP - kl; jl++) {
+        const double *c = cijk + (kl * NP + jl) * NP;
+#pragma unroll
+        for (int il = 0; il <= LP - kl - jl; il++) ci[il] = fma(c[il], w, ci[il]);
+        w *= dj;
+      }
+      dkp *= dk;
+    }
+    const int len = i1 - i0 + 1;
+    const int nch = -wave_min(-((len + kChunk - 1) / kChunk));  // chunks of the longest row
+    int gx = wrap(i0 - lay.shift_local[0], ngx);
+    const int rbase = (int)base;  // level grids hold fewer than 2^31 points
+    double g = 0.0, q = 0.0, di = i0 - T.gp[0];
+    for (int c = 0; c < nch; c++) {
+      const int ib = i0 + c * kChunk;
+      if (c % (kRestart / kChunk) == 0 && ib <= i1) gen_gauss(T, lay, ib, j, k, g, q);
+#pragma unroll
+      for (int s = 0; s < kChunk; s++) {
+        double p = ci[LP];
+#pragma unroll
+        for (int il = LP - 1; il >= 0; il--) p = fma(p, di, ci[il]);
+        const bool in = ib + s <= i1 && gx >= T.blo[0] && gx <= T.bhi[0];
+        S.val[lane * (kChunk + 1) + s] = p * g;
+        S.idx[lane * (kChunk + 1) + s] = in ? rbase + gx : -1;
+        g *= q;
+        q *= r2step;
+        di += 1.0;
+        gx = (gx + 1 == ngx) ? 0 : gx + 1;
+      }
+      wave_sync();
+#pragma unroll
+      for (int h = 0; h < kChunk; h++) {
+        const int e = (h * (64 / kChunk) + lane / kChunk) * (kChunk + 1) + lane % kChunk;
+        const int x = S.idx[e];
+        if (x >= 0) atomicAdd(&grid[x], S.val[e]);
+      }
+      wave_sync();
+    }
+  }
+}
+
+// Integrate: lane-private acc[(kl, jl, il)] (packed, il fastest) of the
+// task's rows; R is the type the potential is read in.
+template <int LP, typename R>
+__device__ __forceinline__ void gen_integrate_rows(const Task &T, const Layout &lay,
+                                                   const R *grid, double *acc, int lane) {
+  const int nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
+  const int ngx = lay.npts_global[0];
+  double hh = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
+  const double r2step = exp(-2.0 * T.zetp * hh);
+  for (int r = lane; r < nj * nk; r += 64) {
+    const int j = T.lo[1] + r % nj, k = T.lo[2] + r / nj;
+    int i0, i1;
+    long long base;
+    if (!gen_row(T, lay, j, k, i0, i1, base)) continue;
+    double ci[LP + 1];
+#pragma unroll
+    for (int il = 0; il <= LP; il++) ci[il] = 0.0;
+    int gx = wrap(i0 - lay.shift_local[0], ngx);
+    for (int s0 = i0; s0 <= i1; s0 += kRestart) {
+      double g, q;
+      gen_gauss(T, lay, s0, j, k, g, q);
+      const int s1 = min(i1, s0 + kRestart - 1);
+      double di = s0 - T.gp[0];
+      for (int i = s0; i <= s1; i++) {
+        double t = 0.0;
+        if (gx >= T.blo[0] && gx <= T.bhi[0]) t = (double)grid[base + gx] * g;
+#pragma unroll
+        for (int il = 0; il <= LP; il++) {
+          ci[il] += t;
+          t *= di;
+        }
+        g *= q;
+        q *= r2step;
+        di += 1.0;
+        gx = (gx + 1 == ngx) ? 0 : gx + 1;
+      }
+    }
+    const double dj = j - T.gp[1], dk = k - T.gp[2];
+    double dkp = 1.0;
+    int e = 0;
+#pragma unroll
+    for (int kl = 0; kl <= LP; kl++) {
+      double w = dkp;
+#pragma unroll
+      for (int jl = 0; jl <= LP - kl; jl++) {
+#pragma unroll
+        for (int il = 0; il <= LP - kl - jl; il++, e++) acc[e] = fma(ci[il], w, acc[e]);
+        w *= dj;
+      }
+      dkp *= dk;
+    }
+  }
+}
+
+template <int LP, typename R>
+__device__ __forceinline__ void gen_integrate_task(const PointParams &P, const Task &T,
+                                                   const Layout &lay, const R *grid,
+                                                   const GenLds &W, int NP, long long task,
+                                                   int lane) {
+  constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
+  double acc[NC];
+#pragma unroll
+  for (int e = 0; e < NC; e++) acc[e] = 0.0;
+  gen_integrate_rows<LP, R>(T, lay, grid, acc, lane);
+#pragma unroll
+  for (int e = 0; e < NC; e++) {
+    const double s = wave_total(acc[e]);
+    if (lane == 63) W.cijk[e] = s;  // packed (il, jl, kl)
+  }
+  wave_sync();
+  // cxyz[i] = sum_o M[o][i] cijk[o], one output per lane
+  const double *M = P.gmat + P.gmat_off[T.level * 8 + LP];
+  double *out = P.cxyz_all + task * P.cxyz_stride;
+  for (int i = lane; i < NC; i += 64) {
+    double s = 0.0;
+    for (int o = 0; o < NC; o++) s = fma(M[o * NC + i], W.cijk[o], s);
+    out[i] = s;
+  }
+}
+
+}  // namespace
+
+// Collocation of general tasks, one task per wave, into the HBM level grids.
+template <int LPB>
+__global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__restrict__ tasks,
+                                                                const Level *__restrict__ levels,
+                                                                PointParams P) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= P.ntasks) return;
+  const long long task = P.task_base + t;
+  const Task &T = tasks[task];
+  const Level &L = levels[T.level];
+  const Layout &lay = L.lay;
+  const int NP = P.wp.np;
+  const GenLds W = gen_lds(lds, P.wp, wave);
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  gen_cijk(P, T, task, lp, W, NP, lane);
+  const GenStage S = gen_stage(W, NP);
+  switch (lp) {
+    case 0: gen_collocate_rows<0>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 1: gen_collocate_rows<1>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 2: gen_collocate_rows<2>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 3: gen_collocate_rows<3>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 4: gen_collocate_rows<4>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 5: if (LPB >= 5) gen_collocate_rows<(LPB >= 5 ? 5 : 0)>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 6: if (LPB >= 6) gen_collocate_rows<(LPB >= 6 ? 6 : 0)>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    default: break;  // the host routes larger lp to the per-task kernels
+  }
+}
+
+// Integration of general tasks, one task per wave; writes the task's cxyz
+// (packed) for the cxyz -> hab pass.  R = float reads Level::gridf.
+template <int LPB, typename R>
+__global__ __launch_bounds__(256) void general_integrate_kernel(const Task *__restrict__ tasks,
+                                                                const Level *__restrict__ levels,
+                                                                PointParams P) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= P.ntasks) return;
+  const long long task = P.task_base + t;
+  const Task &T = tasks[task];
+  const Level &L = levels[T.level];
+  const Layout &lay = L.lay;
+  const int NP = P.wp.np;
+  const GenLds W = gen_lds(lds, P.wp, wave);
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  const R *grid;
+  if constexpr (sizeof(R) == 4)
+    grid = L.gridf;
+  else
+    grid = L.grid;
+  switch (lp) {
+    case 0: gen_integrate_task<0, R>(P, T, lay, grid, W, NP, task, lane); break;
+    case 1: gen_integrate_task<1, R>(P, T, lay, grid, W, NP, task, lane); break;
+    case 2: gen_integrate_task<2, R>(P, T, lay, grid, W, NP, task, lane); break;
+    case 3: gen_integrate_task<3, R>(P, T, lay, grid, W, NP, task, lane); break;
+    case 4: gen_integrate_task<4, R>(P, T, lay, grid, W, NP, task, lane); break;
+    case 5: if (LPB >= 5) gen_integrate_task<(LPB >= 5 ? 5 : 0), R>(P, T, lay, grid, W, NP, task, lane); break;
+    case 6: if (LPB >= 6) gen_integrate_task<(LPB >= 6 ? 6 : 0), R>(P, T, lay, grid, W, NP, task, lane); break;
+    default: break;  // the host routes larger lp to the per-task kernels
+  }
+}
+
+template <typename K>
+static hipError_t launch_gen(K kernel, const PointParams &P, hipStream_t s) {
+  const size_t lds = (size_t)P.wp.bytes;
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const int blocks = (P.ntasks + P.wp.nwaves - 1) / P.wp.nwaves;
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * P.wp.nwaves), lds, s, P.tasks, P.levels, P);
+  return hipGetLastError();
+}
+
+// General tasks [task_base, task_base + ntasks) with lp <= lpb (4 or 6);
+// P.wp: np = lp bound + 1, two dense np^3 arrays per wave.
+hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.np > lpb + 1) return hipErrorInvalidValue;
+  if (lpb == 4) return launch_gen(general_collocate_kernel<4>, P, s);
+  if (lpb == 6) return launch_gen(general_collocate_kernel<6>, P, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.np > lpb + 1) return hipErrorInvalidValue;
+  if (lpb == 4)
+    return P.fp32 ? launch_gen(general_integrate_kernel<4, float>, P, s)
+                  : launch_gen(general_integrate_kernel<4, double>, P, s);
+  if (lpb == 6)
+    return P.fp32 ? launch_gen(general_integrate_kernel<6, float>, P, s)
+                  : launch_gen(general_integrate_kernel<6, double>, P, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace gm

@@ -160,6 +160,11 @@ struct PointParams {
   int unit_base;          // first unit of the launch (one unit per workgroup)
   int lane_cxyz;          // cab -> cxyz of GRID_FUNC_AB tasks with la, lb <= 2 by lane
   int fp32;               // FP32-mixed: point evaluation in FP32, accumulation in FP64
+  // general cells: per level and lp the matrix M[o][i] of the change of
+  // variables cxyz (i, packed (lx, ly, lz)) -> cijk (o, packed (il, jl, kl)),
+  // followed by its transpose; block (level, lp) at gmat + gmat_off[level * 8 + lp]
+  const double *gmat;
+  const int *gmat_off;
 };
 
 }  // namespace gm

@@ -39,65 +39,6 @@
 namespace gm {
 namespace {
 
-// LDS written by one lane and read by another lane of the same wave: LDS
-// requests of a wave complete in order, so only compiler reordering must be
-// prevented.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_mov(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, true);
-  return __hiloint2double(hi, lo);
-}
-
-// Sum over the 64 lanes (all active); the total is valid in lane 63.
-__device__ __forceinline__ double wave_total(double v) {
-  v += dpp_mov<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
-  v += dpp_mov<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
-  v += dpp_mov<0x141, 0xf>(v);  // row_half_mirror
-  v += dpp_mov<0x140, 0xf>(v);  // row_mirror
-  v += dpp_mov<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_mov<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-  return v;
-}
-
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-__device__ __forceinline__ int pm(int a, int m) {
-  const int r = a % m;
-  return r < 0 ? r + m : r;
-}
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// a mod n for a within a few periods of [0, n).
-__device__ __forceinline__ int wrap(int a, int n) {
-  while (a < 0) a += n;
-  while (a >= n) a -= n;
-  return a;
-}
-
-// Packed index of (lx, ly, lz), lx + ly + lz <= lp: lz slowest, lx fastest.
-__device__ __forceinline__ int packed_index(int lx, int ly, int lz, int lp) {
-  int off = 0;
-  for (int z = 0; z < lz; z++) off += (lp - z + 1) * (lp - z + 2) / 2;
-  for (int y = 0; y < ly; y++) off += lp - lz - y + 1;
-  return off + lx;
-}
-
-// Multiply-add in the evaluation type: FP64, or FP32 in FP32-mixed mode.
-__device__ __forceinline__ double fmar(double a, double b, double c) { return fma(a, b, c); }
-__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
 // Per-wave LDS region.
 struct WaveLds {
   double *cxyz;  // dense [np][np][np] (setup, finish) or packed (collocate points)

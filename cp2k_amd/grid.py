@@ -43,10 +43,11 @@ FP32_MIXED = 1
 
 KERNEL_COLLOCATE, KERNEL_INTEGRATE, KERNEL_DECONTRACT, KERNEL_CONTRACT, \
     KERNEL_FILL, KERNEL_REDUCE, KERNEL_CXYZ, KERNEL_TILES, KERNEL_DIRECT, \
-    KERNEL_INTEGRATE_POINTS, KERNEL_FINISH, KERNEL_PER_TASK, KERNEL_TO_FLOAT = range(13)
+    KERNEL_INTEGRATE_POINTS, KERNEL_FINISH, KERNEL_PER_TASK, KERNEL_TO_FLOAT, \
+    KERNEL_GENERAL_POINTS = range(14)
 KERNEL_NAMES = ["collocate", "integrate", "decontract", "contract", "fill", "reduce",
                 "cxyz", "collocate_tiles", "collocate_direct", "integrate_points",
-                "finish", "per_task", "to_float"]
+                "finish", "per_task", "to_float", "general_points"]
 
 
 class GridBasisSet(C.Structure):

@@ -99,12 +99,13 @@ void *grid_mi355x_get_stream(void);
 void grid_mi355x_synchronize(void);
 
 /* Per-kernel timing with HIP events on the launch stream.  Kernel ids:
- * 0 collocate (whole call: ids 6, 7, 8 and 11 of it), 1 integrate (whole
- * call: ids 9, 10, 11 of it), 2 decontract pab, 3 contract hab, 4 zero/fill,
+ * 0 collocate (whole call: ids 6, 7, 8, 11 and 13 of it), 1 integrate (whole
+ * call: ids 9, 10, 11, 13 of it), 2 decontract pab, 3 contract hab, 4 zero/fill,
  * 5 force/virial reduction, 6 cab -> cxyz pass, 7 collocation through LDS
  * tiles, 8 collocation straight into HBM, 9 integrate points, 10 cxyz ->
- * hab/forces pass, 11 per-task kernels (general cells, lp > 6), 12 rounding
- * of the potential to float (FP32-mixed integrate). */
+ * hab/forces pass, 11 per-task kernels (lp > 6), 12 rounding of the
+ * potential to float (FP32-mixed integrate), 13 general-cell point kernels
+ * (collocate or integrate). */
 void grid_mi355x_set_timing(const bool enable);
 void grid_mi355x_reset_timing(void);
 /* Returns total milliseconds and number of launches of kernel id. */
