@@ -40,6 +40,8 @@ hipError_t launch_fill(double *p, long long n, double v, hipStream_t s);
 hipError_t launch_to_float(const double *src, float *dst, long long n, hipStream_t s);
 hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *task_out,
                                 double *forces, int natoms, hipStream_t s);
+hipError_t launch_reduce_virial_lists(const int *ptr, const int *ent, const double *task_out,
+                                     double *virial, int nlists, hipStream_t s);
 hipError_t launch_reduce_virial(const double *task_out, long long ntasks, double *virial,
                                 hipStream_t s);
 hipError_t launch_cxyz_wave(const PointParams &P, hipStream_t s);
@@ -84,7 +86,7 @@ static inline int pmod(int a, int m) { return ((a % m) + m) % m; }
 // Library state (src/grid/common/grid_library.c)
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kNumKernelIds = 14;
+constexpr int kNumKernelIds = 15;
 constexpr int kCounterLp = 20;
 
 struct PendingTiming {
@@ -100,6 +102,7 @@ struct LibState {
   hipStream_t user_stream = nullptr;
   bool stream_set = false;
   int precision = -1;
+  bool capturing = false;  // a batch graph is being captured: no timing events
   int wave_kernels = -1;  // wave-per-task kernels for orthorhombic tasks
   bool timing = false;
   double kernel_ms[kNumKernelIds] = {};
@@ -150,7 +153,7 @@ struct TimedLaunch {
   hipEvent_t a = nullptr, b = nullptr;
   hipStream_t s;
   TimedLaunch(int id_, hipStream_t s_) : id(id_), s(s_) {
-    if (S.timing && id >= 0) {
+    if (S.timing && !S.capturing && id >= 0) {
       a = get_event();
       b = get_event();
       HIP_CHECK(hipEventRecord(a, s));
@@ -327,6 +330,28 @@ struct TilePlan {
   std::vector<std::array<std::pair<int, int>, 2>> range;  // [level][bucket] = (first, count)
 };
 
+// Launch class of a task in merged mode.
+enum TaskClass : signed char { kClsOrthoClassed = 0, kClsOrthoOther = 1, kClsGeneral = 2, kClsSkip = 3 };
+
+// Merged launches for one lp shift: task index lists of the launch kinds.
+enum MergedList { kMSetup, kMOrthoA, kMOrthoB, kMGenA, kMGenB, kMOrthoRest, kMGenRest, kMNum };
+struct MergedPlan {
+  DevBuf d;
+  int off[kMNum + 1] = {};  // list m = d[off[m] .. off[m + 1])
+  int lp[kMNum] = {};       // largest lp (with the shift) in list m
+};
+
+// Arguments of grid_create_task_list, kept for building batched lists.
+struct ListInputs {
+  bool ortho = false;
+  int ntasks = 0, nlevels = 0, natoms = 0, nkinds = 0, nblocks = 0;
+  std::vector<int> block_offsets, atom_kinds, level, iatom, jatom, iset, jset, ipgf, jpgf,
+      border_mask, block_num, npts_global, npts_local, shift_local, border_width;
+  std::vector<double> atom_positions, radius, rab, dh, dh_inv;
+  std::vector<const grid_basis_set *> basis_sets;
+  std::vector<char> level_ortho;  // per level (batched lists of mixed cells)
+};
+
 // Largest LDS tile in doubles; the rest of LDS holds the wave regions.
 constexpr long long kTileDoubles = 14336;
 
@@ -375,6 +400,14 @@ struct TaskListImpl {
   long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
   std::vector<OrthoLevel> olev;   // per level
   std::vector<OrthoLevel> glev;   // per level: general tasks (begin, end, lp_start)
+  std::vector<signed char> cls;   // per sorted task: a TaskClass
+  std::vector<int> task_level;    // per sorted task: grid level (zero based)
+  int max_ext_all = 0;            // largest cube extent of the orthorhombic tasks
+  // merged launches (batched lists): one launch per kind over all levels
+  bool merged = false;
+  std::map<int, MergedPlan> merged_plans;  // by lp shift
+  const Level *static_levels = nullptr;    // fixed level table (graph capture)
+  ListInputs inputs;                       // the create arguments (for batching)
   std::vector<signed char> lp0s;  // lp0 per sorted task
   std::vector<int> highl_before;  // sorted tasks before s with la_max or lb_max > 2
   int ngeo = 0;
@@ -392,6 +425,7 @@ struct TaskListImpl {
                       &d_geo, &d_geo_cols, &d_geo_kmin, &d_cxyz, &d_geo_rows, &d_items,
                       &d_gridf, &d_gmat, &d_gmat_off};
     for (auto &kv : tile_plans) kv.second.d.release();
+    for (auto &kv : merged_plans) kv.second.d.release();
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
     for (LevelSlot &ls : level_slots) {
@@ -426,7 +460,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
                      const int *border_mask_list, const int *block_num_list,
                      const double *radius_list, const double *rab_list,
                      const int *npts_global, const int *npts_local, const int *shift_local,
-                     const int *border_width, const double *dh, const double *dh_inv) {
+                     const int *border_width, const double *dh, const double *dh_inv,
+                     const char *level_ortho = nullptr) {
   const grid_library_config cfg = grid_library_get_config();
   if (cfg.backend != GRID_BACKEND_AUTO && cfg.backend != GRID_BACKEND_GPU)
     fatal("grid backend %d is not provided by this library (only AUTO/GPU, the "
@@ -441,6 +476,40 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->nkinds = nkinds;
   tl->nblocks = nblocks;
   tl->npts_local.assign(npts_local, npts_local + 3 * (size_t)std::max(nlevels, 0));
+  {
+    ListInputs &in = tl->inputs;
+    auto cp = [](auto &v, const auto *p, size_t n) { v.assign(p, p + (p ? n : 0)); };
+    in.ortho = orthorhombic;
+    in.ntasks = ntasks;
+    in.nlevels = nlevels;
+    in.natoms = natoms;
+    in.nkinds = nkinds;
+    in.nblocks = nblocks;
+    const size_t nt = (size_t)std::max(ntasks, 0), nl = (size_t)std::max(nlevels, 0);
+    cp(in.block_offsets, block_offsets, (size_t)std::max(nblocks, 0));
+    cp(in.atom_positions, atom_positions, 3 * (size_t)std::max(natoms, 0));
+    cp(in.atom_kinds, atom_kinds, (size_t)std::max(natoms, 0));
+    cp(in.basis_sets, basis_sets, (size_t)std::max(nkinds, 0));
+    cp(in.level, level_list, nt);
+    cp(in.iatom, iatom_list, nt);
+    cp(in.jatom, jatom_list, nt);
+    cp(in.iset, iset_list, nt);
+    cp(in.jset, jset_list, nt);
+    cp(in.ipgf, ipgf_list, nt);
+    cp(in.jpgf, jpgf_list, nt);
+    cp(in.border_mask, border_mask_list, nt);
+    cp(in.block_num, block_num_list, nt);
+    cp(in.radius, radius_list, nt);
+    cp(in.rab, rab_list, 3 * nt);
+    cp(in.npts_global, npts_global, 3 * nl);
+    cp(in.npts_local, npts_local, 3 * nl);
+    cp(in.shift_local, shift_local, 3 * nl);
+    cp(in.border_width, border_width, 3 * nl);
+    cp(in.dh, dh, 9 * nl);
+    cp(in.dh_inv, dh_inv, 9 * nl);
+    if (level_ortho) in.level_ortho.assign(level_ortho, level_ortho + nl);
+    else in.level_ortho.assign(nl, orthorhombic ? 1 : 0);
+  }
   // src/grid/grid_task_list.c:65-70
   tl->empty = (nblocks == 0 || ntasks == 0 || nlevels == 0);
   if (tl->empty) return;
@@ -655,7 +724,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
       tl->n_skip++;
       continue;
     }
-    if (orthorhombic && bm == 0) {
+    if ((level_ortho ? level_ortho[level] != 0 : orthorhombic) && bm == 0) {
       // ortho_cxyz_to_grid, grid_ref_collint.h:222-257
       T.flags = kTaskOrtho;
       for (int d = 0; d < 3; d++) {
@@ -794,6 +863,18 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->class_npts.assign(ntasks, 0);
   for (int s = 0; s < ntasks; s++)
     if (tasks[s].geo >= 0) tl->class_npts[s] = geo[tasks[s].geo].npts;
+  tl->cls.resize(ntasks);
+  tl->task_level.resize(ntasks);
+  for (int s = 0; s < ntasks; s++) tl->task_level[s] = tasks[s].level;
+  tl->max_ext_all = 0;
+  for (int s = 0; s < ntasks; s++) {
+    const Task &T = tasks[s];
+    tl->cls[s] = (T.flags & kTaskSkip) ? kClsSkip
+                 : (T.flags & kTaskOrtho) ? (T.geo >= 0 ? kClsOrthoClassed : kClsOrthoOther)
+                                          : kClsGeneral;
+    if (T.flags & kTaskOrtho)
+      for (int d = 0; d < 3; d++) tl->max_ext_all = std::max(tl->max_ext_all, 2 - 2 * T.lo[d]);
+  }
 
   // --- per-level ranges of the orthorhombic tasks ---
   tl->olev.assign(nlevels, OrthoLevel{});
@@ -825,6 +906,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->tiles.clear();
   for (auto &kv : tl->tile_plans) kv.second.d.release();
   tl->tile_plans.clear();
+  for (auto &kv : tl->merged_plans) kv.second.d.release();
+  tl->merged_plans.clear();
   for (int l = 0; l < nlevels; l++) {
     OrthoLevel &o = tl->olev[l];
     o.tiled = false;
@@ -1051,12 +1134,13 @@ enum LaunchMode { kCollocate = 0, kIntegrate = 1, kCount = 2 };
 // whole level grid in LDS when it fits next to at least four wave regions.
 enum PlanKind { kPlanSetup, kPlanPoints, kPlanWindow };
 
+// level -1: a merged launch over all levels (not for kPlanWindow).
 WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[4], int kind) {
-  const OrthoLevel &o = tl->olev[level];
+  const OrthoLevel &o = tl->olev[std::max(level, 0)];
   WavePlan w{};
   w.level = level;
   w.np = lp + 1;
-  w.ext = o.max_ext;
+  w.ext = level < 0 ? tl->max_ext_all : o.max_ext;
   int nd;
   if (kind == kPlanSetup) {
     // cab [ncoset(la)][ncoset(lb)] + alpha [3][lb+1][la+1][lp+1] for la + lb <= lp
@@ -1285,9 +1369,108 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   }
 }
 
+// Task index lists of the merged launches for lp shift sh.
+MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
+  auto it = tl->merged_plans.find(sh);
+  if (it != tl->merged_plans.end()) return it->second;
+  MergedPlan &plan = tl->merged_plans[sh];
+  std::vector<int> lists[kMNum];
+  const bool wave = wave_kernels();
+  for (int t = 0; t < tl->ntasks; t++) {
+    const int lp = tl->lp0s[t] + sh;
+    const int c = tl->cls[t];
+    if (c == kClsSkip) continue;
+    if (c == kClsGeneral) {
+      if (wave && lp <= 6) {
+        lists[kMSetup].push_back(t);
+        lists[lp <= 4 ? kMGenA : kMGenB].push_back(t);
+      } else {
+        lists[kMGenRest].push_back(t);
+      }
+    } else if (c == kClsOrthoClassed && wave && lp <= 6) {
+      lists[kMSetup].push_back(t);
+      lists[lp <= 4 ? kMOrthoA : kMOrthoB].push_back(t);
+    } else {
+      lists[kMOrthoRest].push_back(t);
+    }
+  }
+  std::vector<int> all;
+  for (int m = 0; m < kMNum; m++) {
+    plan.off[m] = (int)all.size();
+    int mx = 0;
+    for (int t : lists[m]) mx = std::max(mx, tl->lp0s[t] + sh);
+    plan.lp[m] = mx;
+    all.insert(all.end(), lists[m].begin(), lists[m].end());
+  }
+  plan.off[kMNum] = (int)all.size();
+  upload(plan.d, all);
+  return plan;
+}
+
+// Batched lists: every launch kind once over all levels (task index lists);
+// collocation adds straight into HBM (no LDS tiles).
+void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+  const int sh = P.ld[0] + P.ld[2];
+  MergedPlan &mp = merged_plan(tl, sh);
+  const int *idx = (const int *)mp.d.p;
+  auto sub = [&](int m) {
+    PointParams Q = P;
+    Q.task_idx = idx + mp.off[m];
+    Q.task_base = 0;
+    Q.ntasks = mp.off[m + 1] - mp.off[m];
+    return Q;
+  };
+  PointParams Q = sub(kMSetup);
+  if (Q.ntasks > 0) {
+    Q.wp = make_wave_plan(tl, -1, std::min(6, mp.lp[kMSetup]), P.ld, kPlanSetup);
+    if (mode == kCollocate) {
+      TimedLaunch tm(6, s);
+      const bool ab = P.func == 100 && P.ld[0] == 0 && P.ld[1] == 0 && P.ld[2] == 0 &&
+                      P.ld[3] == 0;
+      Q.lane_cxyz = ab ? 1 : 0;
+      if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
+      HIP_CHECK(launch_cxyz_wave(Q, s));
+    }
+    for (int m : {kMOrthoA, kMOrthoB}) {
+      PointParams R = sub(m);
+      if (R.ntasks <= 0) continue;
+      const int lpb = m == kMOrthoA ? 4 : 6;
+      R.wp = make_wave_plan(tl, -1, std::min(lpb, mp.lp[m]), P.ld, kPlanPoints);
+      TimedLaunch tm(mode == kIntegrate ? 9 : 8, s);
+      HIP_CHECK(mode == kIntegrate ? launch_integrate_wave(R, lpb, s)
+                                   : launch_collocate_wave(R, lpb, s));
+    }
+    for (int m : {kMGenA, kMGenB}) {
+      PointParams R = sub(m);
+      if (R.ntasks <= 0) continue;
+      const int lpb = m == kMGenA ? 4 : 6;
+      R.wp = general_plan(std::min(lpb, mp.lp[m]));
+      TimedLaunch tm(13, s);
+      HIP_CHECK(mode == kIntegrate ? launch_general_integrate(R, lpb, s)
+                                   : launch_general_collocate(R, lpb, s));
+    }
+    if (mode == kIntegrate) {
+      TimedLaunch tm(10, s);
+      Q.lane_cxyz = 0;
+      HIP_CHECK(launch_finish_wave(Q, s));
+    }
+  }
+  for (int m : {kMOrthoRest, kMGenRest}) {
+    PointParams R = sub(m);
+    if (R.ntasks <= 0) continue;
+    TimedLaunch tm(11, s);
+    const bool ortho = m == kMOrthoRest;
+    HIP_CHECK(mode == kIntegrate ? launch_integrate(R, ortho, s) : launch_collocate(R, ortho, s));
+  }
+}
+
 // Launches the orthorhombic and the general task ranges (tasks are sorted
 // by class: orthorhombic, general, skipped).
 void launch_ranges(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
+  if (tl->merged && mode != kCount) {
+    launch_merged(tl, P, s, mode);
+    return;
+  }
   if (mode == kCount) {
     // tasks with a geometry class are counted on the host
     for (int l = 0; l < tl->nlevels; l++) {
@@ -1332,6 +1515,7 @@ long long level_points(const TaskListImpl *tl, int l) {
 // potential rounded to FP32 sits at gridf + (points of the levels before l).
 const Level *set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s,
                         const float *gridf = nullptr) {
+  if (tl->static_levels) return tl->static_levels;
   TaskListImpl::LevelSlot &ls = tl->level_slots[tl->next_level_slot];
   tl->next_level_slot = (tl->next_level_slot + 1) % TaskListImpl::kLevelSlots;
   if (!ls.h) {
@@ -1359,6 +1543,7 @@ const Level *set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s,
 // After the last kernel of a call: the slot is free once that kernel is done
 // (also when the next call of the ring comes on another stream).
 void end_levels(TaskListImpl *tl, hipStream_t s) {
+  if (tl->static_levels) return;
   HIP_CHECK(hipEventRecord(tl->level_slots[tl->last_level_slot].copied, s));
 }
 
@@ -1417,9 +1602,16 @@ void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const 
   count_stats(tl, true, ld[0] + ld[2]);
 }
 
+// Per-list virial of a batched list: virial[l * 9 ..] sums the tasks
+// ent[ptr[l] .. ptr[l + 1]).
+struct ListVirial {
+  const int *ptr, *ent;
+  int nlists;
+};
+
 void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *const *grids,
                    double *hab, long long hab_len, double *forces, double *virial,
-                   hipStream_t s) {
+                   hipStream_t s, const ListVirial *lv = nullptr) {
   const bool do_f = forces != nullptr, do_v = virial != nullptr;
   if (do_v && !do_f)
     fatal("virial requires forces (src/grid/ref/grid_ref_integrate.c:109)");
@@ -1488,7 +1680,9 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
       HIP_CHECK(launch_reduce_forces((const int *)tl->d_fptr.p, (const int *)tl->d_fent.p,
                                      (const double *)tl->d_task_out.p, forces, tl->natoms, s));
     if (do_v)
-      HIP_CHECK(launch_reduce_virial((const double *)tl->d_task_out.p, tl->ntasks, virial, s));
+      HIP_CHECK(lv ? launch_reduce_virial_lists(lv->ptr, lv->ent, (const double *)tl->d_task_out.p,
+                                                virial, lv->nlists, s)
+                   : launch_reduce_virial((const double *)tl->d_task_out.p, tl->ntasks, virial, s));
   }
   end_levels(tl, s);
   count_stats(tl, false, ld[0] + ld[2]);
@@ -1511,6 +1705,82 @@ double *device_mirror(const offload_buffer *b, DevBuf &scratch) {
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Batched task lists: the lists are merged into one list (levels, atoms,
+// kinds, blocks and buffers concatenated) whose launches each cover every
+// list, and a run replays a hipGraph of the merged calls.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BatchImpl {
+  static constexpr unsigned kMagic = 0x42415443u;
+  unsigned magic = kMagic;
+  int nlists = 0;
+  TaskListImpl *merged = nullptr;
+  std::vector<int> level_off, atom_off;  // per list (+ total)
+  std::vector<long long> pab_off;        // per list (+ total), doubles
+  std::vector<long long> grid_off;       // per merged level (+ total), doubles
+  DevBuf pab, hab, rho, pot, forces, virial, vptr, vent, lev_col, lev_int;
+  std::vector<double *> rho_lv, pot_lv;  // per merged level
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<std::array<int, 5>, Graph> graphs;  // (what, func, tau, forces, precision)
+  ~BatchImpl() {
+    for (auto &kv : graphs)
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    DevBuf *bufs[] = {&pab, &hab, &rho, &pot, &forces, &virial, &vptr, &vent, &lev_col, &lev_int};
+    for (DevBuf *b : bufs) b->release();
+    delete merged;
+    magic = 0;
+  }
+};
+
+BatchImpl *checked_batch(const void *p) {
+  BatchImpl *b = (BatchImpl *)p;
+  if (!b || b->magic != BatchImpl::kMagic) fatal("invalid or freed batch handle");
+  return b;
+}
+
+// Level table with fixed grid pointers, uploaded synchronously.
+void static_levels(BatchImpl *b, DevBuf &dst, const std::vector<double *> &grids,
+                   const float *gridf) {
+  TaskListImpl *tl = b->merged;
+  std::vector<Level> lv(tl->nlevels);
+  long long off = 0;
+  for (int l = 0; l < tl->nlevels; l++) {
+    lv[l].lay = tl->layouts[l];
+    lv[l].grid = grids[l];
+    lv[l].npts = level_points(tl, l);
+    lv[l].gridf = gridf ? gridf + off : nullptr;
+    off += lv[l].npts;
+  }
+  upload(dst, lv);
+}
+
+// The merged calls of a run: per batch, collocate (what & 1) then integrate
+// (what & 2).
+void batch_enqueue(BatchImpl *b, int what, int func, bool tau, bool forces, bool fixed,
+                   hipStream_t s) {
+  TaskListImpl *tl = b->merged;
+  double *pab = (double *)b->pab.p;
+  if (what & 1) {
+    tl->static_levels = fixed ? (const Level *)b->lev_col.p : nullptr;
+    run_collocate(tl, func, pab, b->rho_lv.data(), s);
+  }
+  if (what & 2) {
+    tl->static_levels = fixed ? (const Level *)b->lev_int.p : nullptr;
+    const ListVirial lv{(const int *)b->vptr.p, (const int *)b->vent.p, b->nlists};
+    run_integrate(tl, tau, forces ? pab : nullptr, b->pot_lv.data(), (double *)b->hab.p,
+                  b->pab_off[b->nlists], forces ? (double *)b->forces.p : nullptr,
+                  forces ? (double *)b->virial.p : nullptr, s, forces ? &lv : nullptr);
+  }
+  tl->static_levels = nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 // ---- offload_buffer, src/offload/offload_buffer.c:52-118 ----
@@ -2004,6 +2274,169 @@ void grid_mi355x_task_counts(const grid_task_list *task_list, long *ortho, long 
   if (ortho) *ortho = tl->n_ortho;
   if (general) *general = tl->n_general;
   if (skipped) *skipped = tl->n_skip;
+}
+
+
+// ---- batched task lists (no reference counterpart) ----
+void grid_mi355x_create_batch(const int nlists, const grid_task_list *const *lists,
+                              grid_mi355x_batch **batch) {
+  if (nlists <= 0) fatal("a batch needs at least one task list");
+  BatchImpl *b = new BatchImpl();
+  b->nlists = nlists;
+  ListInputs all;
+  b->level_off.assign(1, 0);
+  b->atom_off.assign(1, 0);
+  b->pab_off.assign(1, 0);
+  int kind_off = 0, block_off = 0;
+  bool any_ortho = false;
+  for (int i = 0; i < nlists; i++) {
+    const TaskListImpl *tl = checked(lists[i]);
+    const ListInputs &in = tl->inputs;
+    const int lo = b->level_off.back(), ao = b->atom_off.back();
+    const long long po = b->pab_off.back();
+    if (po + tl->block_end >= (1LL << 31)) fatal("batch: pab buffers above 2^31 doubles");
+    for (int v : in.block_offsets) all.block_offsets.push_back((int)(po + v));
+    all.atom_positions.insert(all.atom_positions.end(), in.atom_positions.begin(),
+                              in.atom_positions.end());
+    for (int k : in.atom_kinds) all.atom_kinds.push_back(k + kind_off);
+    all.basis_sets.insert(all.basis_sets.end(), in.basis_sets.begin(), in.basis_sets.end());
+    for (int t = 0; t < in.ntasks; t++) {
+      all.level.push_back(in.level[t] + lo);
+      all.iatom.push_back(in.iatom[t] + ao);
+      all.jatom.push_back(in.jatom[t] + ao);
+      all.block_num.push_back(in.block_num[t] + block_off);
+    }
+    for (const auto &pr : {std::make_pair(&all.iset, &in.iset), std::make_pair(&all.jset, &in.jset),
+                     std::make_pair(&all.ipgf, &in.ipgf), std::make_pair(&all.jpgf, &in.jpgf),
+                     std::make_pair(&all.border_mask, &in.border_mask),
+                     std::make_pair(&all.npts_global, &in.npts_global),
+                     std::make_pair(&all.npts_local, &in.npts_local),
+                     std::make_pair(&all.shift_local, &in.shift_local),
+                     std::make_pair(&all.border_width, &in.border_width)})
+      pr.first->insert(pr.first->end(), pr.second->begin(), pr.second->end());
+    for (const auto &pr : {std::make_pair(&all.radius, &in.radius), std::make_pair(&all.rab, &in.rab),
+                     std::make_pair(&all.dh, &in.dh), std::make_pair(&all.dh_inv, &in.dh_inv)})
+      pr.first->insert(pr.first->end(), pr.second->begin(), pr.second->end());
+    all.level_ortho.insert(all.level_ortho.end(), in.level_ortho.begin(), in.level_ortho.end());
+    for (char c : in.level_ortho) any_ortho = any_ortho || c;
+    all.ntasks += in.ntasks;
+    kind_off += in.nkinds;
+    block_off += in.nblocks;
+    b->level_off.push_back(lo + in.nlevels);
+    b->atom_off.push_back(ao + in.natoms);
+    b->pab_off.push_back(po + std::max<long long>(tl->block_end, 1));
+  }
+  const int nlevels = b->level_off.back(), natoms = b->atom_off.back();
+  b->merged = new TaskListImpl();
+  TaskListImpl *tl = b->merged;
+  build_task_list(tl, any_ortho, all.ntasks, nlevels, natoms, kind_off, block_off,
+                  all.block_offsets.data(), all.atom_positions.data(), all.atom_kinds.data(),
+                  all.basis_sets.data(), all.level.data(), all.iatom.data(), all.jatom.data(),
+                  all.iset.data(), all.jset.data(), all.ipgf.data(), all.jpgf.data(),
+                  all.border_mask.data(), all.block_num.data(), all.radius.data(),
+                  all.rab.data(), all.npts_global.data(), all.npts_local.data(),
+                  all.shift_local.data(), all.border_width.data(), all.dh.data(),
+                  all.dh_inv.data(), all.level_ortho.data());
+  tl->merged = true;
+  // buffers
+  b->grid_off.assign(1, 0);
+  for (int l = 0; l < nlevels; l++) b->grid_off.push_back(b->grid_off.back() + level_points(tl, l));
+  b->pab.get(sizeof(double) * (size_t)b->pab_off.back());
+  b->hab.get(sizeof(double) * (size_t)b->pab_off.back());
+  b->rho.get(sizeof(double) * (size_t)std::max(b->grid_off.back(), 1LL));
+  b->pot.get(sizeof(double) * (size_t)std::max(b->grid_off.back(), 1LL));
+  b->forces.get(sizeof(double) * 3 * (size_t)std::max(natoms, 1));
+  b->virial.get(sizeof(double) * 9 * (size_t)nlists);
+  const hipStream_t s = stream();
+  HIP_CHECK(launch_fill((double *)b->pab.p, b->pab_off.back(), 0.0, s));
+  HIP_CHECK(launch_fill((double *)b->pot.p, std::max(b->grid_off.back(), 1LL), 0.0, s));
+  for (int l = 0; l < nlevels; l++) {
+    b->rho_lv.push_back((double *)b->rho.p + b->grid_off[l]);
+    b->pot_lv.push_back((double *)b->pot.p + b->grid_off[l]);
+  }
+  // tasks of each list (CSR) for the per-list virial
+  std::vector<int> list_of_level(std::max(nlevels, 1), 0), vptr(nlists + 1, 0), vent;
+  for (int i = 0; i < nlists; i++)
+    for (int l = b->level_off[i]; l < b->level_off[i + 1]; l++) list_of_level[l] = i;
+  if (!tl->empty) {
+    for (int t = 0; t < tl->ntasks; t++) vptr[list_of_level[tl->task_level[t]] + 1]++;
+    for (int i = 0; i < nlists; i++) vptr[i + 1] += vptr[i];
+    vent.resize(tl->ntasks);
+    std::vector<int> fill(vptr.begin(), vptr.end() - 1);
+    for (int t = 0; t < tl->ntasks; t++) vent[fill[list_of_level[tl->task_level[t]]]++] = t;
+  }
+  upload(b->vptr, vptr);
+  upload(b->vent, vent);
+  sync_stream(s);
+  *batch = (grid_mi355x_batch *)b;
+}
+
+void grid_mi355x_free_batch(grid_mi355x_batch *batch) {
+  if (!batch) return;
+  BatchImpl *b = checked_batch(batch);
+  sync_stream(stream());
+  delete b;
+}
+
+void grid_mi355x_batch_buffers(const grid_mi355x_batch *batch, const int list, double **pab,
+                               double **hab, long *length, double **forces, double **virial) {
+  const BatchImpl *b = checked_batch(batch);
+  if (list < 0 || list >= b->nlists) fatal("batch: list %d out of range", list);
+  if (pab) *pab = (double *)b->pab.p + b->pab_off[list];
+  if (hab) *hab = (double *)b->hab.p + b->pab_off[list];
+  if (length) *length = (long)(b->pab_off[list + 1] - b->pab_off[list]);
+  if (forces) *forces = (double *)b->forces.p + 3 * (size_t)b->atom_off[list];
+  if (virial) *virial = (double *)b->virial.p + 9 * (size_t)list;
+}
+
+double *grid_mi355x_batch_grid(const grid_mi355x_batch *batch, const int list, const int level,
+                               const int potential) {
+  const BatchImpl *b = checked_batch(batch);
+  if (list < 0 || list >= b->nlists) fatal("batch: list %d out of range", list);
+  const int l = b->level_off[list] + level;
+  if (level < 0 || l >= b->level_off[list + 1]) fatal("batch: level %d out of range", level);
+  return potential ? b->pot_lv[l] : b->rho_lv[l];
+}
+
+void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what, const enum grid_func func,
+                           const bool compute_tau, const bool forces) {
+  BatchImpl *b = checked_batch(batch);
+  if (what < 1 || what > 3) fatal("batch: what must be 1, 2 or 3");
+  if (b->merged->empty) return;
+  int ld[4];
+  if (!func_ldiffs(func, ld)) fatal("unknown grid_func %d", (int)func);
+  const hipStream_t s = stream();
+  const std::array<int, 5> key = {what, (int)func, compute_tau ? 1 : 0, forces ? 1 : 0,
+                                  precision()};
+  auto it = b->graphs.find(key);
+  if (it == b->graphs.end()) {
+    // an eager run allocates what the calls allocate on first use and
+    // builds the cached launch plans; the capture then holds kernels only
+    batch_enqueue(b, what, func, compute_tau, forces, false, s);
+    sync_stream(s);
+    static_levels(b, b->lev_col, b->rho_lv, nullptr);
+    static_levels(b, b->lev_int, b->pot_lv,
+                  precision() == GRID_MI355X_FP32_MIXED ? (const float *)b->merged->d_gridf.p
+                                                        : nullptr);
+    hipGraph_t graph = nullptr;
+    S.capturing = true;
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch_enqueue(b, what, func, compute_tau, forces, true, s);
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    S.capturing = false;
+    HIP_CHECK(e);
+    BatchImpl::Graph g;
+    HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphDestroy(graph));
+    it = b->graphs.emplace(key, g).first;
+  }
+  TimedLaunch tm(14, s);
+  HIP_CHECK(hipGraphLaunch(it->second.exec, s));
+}
+
+void grid_mi355x_batch_task_points(const grid_mi355x_batch *batch, long long *points) {
+  const BatchImpl *b = checked_batch(batch);
+  *points = grid_mi355x_task_points((const grid_task_list *)b->merged);
 }
 
 }  // extern "C"

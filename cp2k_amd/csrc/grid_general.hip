@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__re
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   const Level &L = levels[T.level];
   const Layout &lay = L.lay;
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void general_integrate_kernel(const Task *__re
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   const Level &L = levels[T.level];
   const Layout &lay = L.lay;

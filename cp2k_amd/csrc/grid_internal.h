@@ -136,6 +136,7 @@ struct PointParams {
   const Task *tasks;
   int task_base;          // first task of the launched range
   int ntasks;             // tasks in the launched range
+  const int *task_idx;    // if set: the launch covers tasks task_idx[0 .. ntasks)
   const Group *groups;
   const Level *levels;
   const double *cart;     // decontracted pab (collocate; integrate with forces)
@@ -166,5 +167,10 @@ struct PointParams {
   const double *gmat;
   const int *gmat_off;
 };
+
+// Task of position t of a launch.
+__device__ __forceinline__ long long launch_task(const PointParams &P, long long t) {
+  return P.task_idx ? (long long)P.task_idx[t] : P.task_base + t;
+}
 
 }  // namespace gm

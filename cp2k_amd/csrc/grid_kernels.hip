@@ -170,7 +170,7 @@ __device__ int build_chunk(const Task &T, const Layout &lay, bool ortho, int kbe
 __device__ __forceinline__ void stage_task(const PointParams &P, Task &sT, Level &sL, int tid,
                                            int nthr) {
   static_assert(sizeof(Task) % 4 == 0 && sizeof(Level) % 4 == 0, "word copies");
-  const int *src = reinterpret_cast<const int *>(&P.tasks[P.task_base + blockIdx.x]);
+  const int *src = reinterpret_cast<const int *>(&P.tasks[launch_task(P, blockIdx.x)]);
   int *dst = reinterpret_cast<int *>(&sT);
   for (int e = tid; e < (int)(sizeof(Task) / 4); e += nthr) dst[e] = src[e];
   __syncthreads();
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
   const Task &T = sT;
   if (T.flags & kTaskSkip) {
     if (P.count_only && tid == 0)
-      P.task_out[(long long)(P.task_base + blockIdx.x) * kTaskOutStride] = 0.0;
+      P.task_out[launch_task(P, blockIdx.x) * kTaskOutStride] = 0.0;
     return;
   }
   const LdsPlan &pl = P.plan;
@@ -420,7 +420,7 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
     kbeg += npl;
   }
   if (P.count_only && tid == 0)
-    P.task_out[(long long)(P.task_base + blockIdx.x) * kTaskOutStride] = counted;
+    P.task_out[launch_task(P, blockIdx.x) * kTaskOutStride] = counted;
 }
 
 // ---------------------------------------------------------------------------
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   const int tid = threadIdx.x, nthr = 64 * NW;
   stage_task(P, sT, sL, tid, nthr);
   const Task &T = sT;
-  double *out = P.task_out + (long long)(P.task_base + blockIdx.x) * kTaskOutStride;
+  double *out = P.task_out + launch_task(P, blockIdx.x) * kTaskOutStride;
   if (T.flags & kTaskSkip) {
     if (P.do_forces && tid < kTaskOutStride) out[tid] = 0.0;
     return;
@@ -954,6 +954,30 @@ hipError_t launch_reduce_forces(const int *ptr, const int *ent, const double *ta
   if (natoms <= 0) return hipSuccess;
   hipLaunchKernelGGL(reduce_forces_kernel, dim3(natoms), dim3(64), 0, s, ptr, ent, task_out,
                      forces, natoms);
+  return hipGetLastError();
+}
+
+// Virial per list of a batched task list: one workgroup per list.
+__global__ __launch_bounds__(64) void reduce_virial_lists_kernel(const int *ptr, const int *ent,
+                                                                 const double *task_out,
+                                                                 double *virial, int nlists) {
+  const int l = blockIdx.x, lane = threadIdx.x;
+  if (l >= nlists) return;
+  double v[9];
+  for (int c = 0; c < 9; c++) v[c] = 0.0;
+  for (int e = ptr[l] + lane; e < ptr[l + 1]; e += 64)
+    for (int c = 0; c < 9; c++) v[c] += task_out[(long long)ent[e] * kTaskOutStride + 6 + c];
+  for (int c = 0; c < 9; c++) {
+    const double x = wave_sum(v[c]);
+    if (lane == 0) virial[9 * l + c] = x;
+  }
+}
+
+hipError_t launch_reduce_virial_lists(const int *ptr, const int *ent, const double *task_out,
+                                      double *virial, int nlists, hipStream_t s) {
+  if (nlists <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_virial_lists_kernel, dim3(nlists), dim3(64), 0, s, ptr, ent, task_out,
+                     virial, nlists);
   return hipGetLastError();
 }
 

@@ -576,7 +576,7 @@ __global__ __launch_bounds__(256) void cxyz_lane_kernel(const Task *__restrict__
                                                         PointParams P) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   if (T.la_max > 2 || T.lb_max > 2) return;  // cxyz_wave_kernel takes these
   const Group &G = groups[T.group];
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   if (P.lane_cxyz && T.la_max <= 2 && T.lb_max <= 2) return;
   const TaskL l = task_l(P, T);
@@ -640,10 +640,10 @@ __global__ __launch_bounds__(256) void collocate_wave_kernel(const Task *__restr
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   const GeoClass &G = P.geo[T.geo];
-  const Level &L = levels[wp.level];
+  const Level &L = levels[T.level];
   const Layout &lay = L.lay;
   const WaveLds W = wave_lds(lds, wp, wave);
   const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
@@ -723,10 +723,10 @@ __global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   const GeoClass &G = P.geo[T.geo];
-  const Level &L = levels[wp.level];
+  const Level &L = levels[T.level];
   const Layout &lay = L.lay;
   const WaveLds W = wave_lds(lds, wp, wave);
   const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(256, (FULL ? 2 : 4)) void finish_wave_kernel(
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
   if (t >= P.ntasks) return;
-  const long long task = P.task_base + t;
+  const long long task = launch_task(P, t);
   const Task &T = tasks[task];
   const TaskL l = task_l(P, T);
   const WaveLds W = wave_lds(lds, P.wp, wave);

@@ -34,6 +34,8 @@ EXPORTS = [
     "grid_mi355x_set_timing", "grid_mi355x_reset_timing",
     "grid_mi355x_kernel_time_ms", "grid_mi355x_task_points",
     "grid_mi355x_task_points_by_lp", "grid_mi355x_task_counts",
+    "grid_mi355x_create_batch", "grid_mi355x_free_batch", "grid_mi355x_batch_buffers",
+    "grid_mi355x_batch_grid", "grid_mi355x_batch_run", "grid_mi355x_batch_task_points",
 ]
 
 GRID_FUNC_AB = 100
@@ -44,10 +46,10 @@ FP32_MIXED = 1
 KERNEL_COLLOCATE, KERNEL_INTEGRATE, KERNEL_DECONTRACT, KERNEL_CONTRACT, \
     KERNEL_FILL, KERNEL_REDUCE, KERNEL_CXYZ, KERNEL_TILES, KERNEL_DIRECT, \
     KERNEL_INTEGRATE_POINTS, KERNEL_FINISH, KERNEL_PER_TASK, KERNEL_TO_FLOAT, \
-    KERNEL_GENERAL_POINTS = range(14)
+    KERNEL_GENERAL_POINTS, KERNEL_BATCH = range(15)
 KERNEL_NAMES = ["collocate", "integrate", "decontract", "contract", "fill", "reduce",
                 "cxyz", "collocate_tiles", "collocate_direct", "integrate_points",
-                "finish", "per_task", "to_float", "general_points"]
+                "finish", "per_task", "to_float", "general_points", "batch"]
 
 
 class GridBasisSet(C.Structure):
@@ -98,6 +100,15 @@ def lib():
         L.grid_mi355x_get_stream.restype = C.c_void_p
         L.grid_mi355x_set_stream.argtypes = [C.c_void_p]
         L.grid_mi355x_set_timing.argtypes = [C.c_bool]
+        L.grid_mi355x_create_batch.argtypes = [C.c_int, C.POINTER(C.c_void_p),
+                                               C.POINTER(C.c_void_p)]
+        L.grid_mi355x_free_batch.argtypes = [C.c_void_p]
+        L.grid_mi355x_batch_buffers.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_void_p)] * 2 + [
+            C.POINTER(C.c_long)] + [C.POINTER(C.c_void_p)] * 2
+        L.grid_mi355x_batch_grid.restype = C.c_void_p
+        L.grid_mi355x_batch_grid.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.grid_mi355x_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_bool, C.c_bool]
+        L.grid_mi355x_batch_task_points.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.grid_mi355x_device_malloc.restype = C.c_void_p
         L.grid_mi355x_device_malloc.argtypes = [C.c_size_t]
         L.grid_mi355x_device_free.argtypes = [C.c_void_p]
@@ -331,3 +342,101 @@ def kernel_time_ms(kernel_id):
     n = C.c_long()
     ms = lib().grid_mi355x_kernel_time_ms(C.c_int(kernel_id), C.byref(n))
     return ms, n.value
+
+
+class _DevView:
+    """A device range owned by someone else (a batch buffer), shaped."""
+
+    def __init__(self, ptr, shape):
+        self.ptr = int(ptr)
+        self.shape = tuple(shape)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = 8 * self.size
+
+    def data_ptr(self):
+        return self.ptr
+
+    def numel(self):
+        return self.size
+
+    def numpy(self):
+        a = np.empty(self.shape)
+        lib().grid_mi355x_memcpy(a.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                 C.c_size_t(self.nbytes), C.c_int(2))
+        return a
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.size:
+            raise ValueError(f"{a.size} values for a buffer of {self.size}")
+        lib().grid_mi355x_memcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p),
+                                 C.c_size_t(self.nbytes), C.c_int(1))
+
+
+class GridBatch:
+    """Several GridTaskLists merged into one batch (grid_mi355x_create_batch):
+    every launch covers all lists, a run replays a captured hipGraph.  The
+    batch owns the HBM buffers of each list (pab/hab, grids, forces, virial)."""
+
+    def __init__(self, task_lists):
+        self.lists = list(task_lists)
+        arr = (C.c_void_p * len(self.lists))(*[t.h.value for t in self.lists])
+        self.h = C.c_void_p()
+        lib().grid_mi355x_create_batch(C.c_int(len(self.lists)), arr, C.byref(self.h))
+
+    def free(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().grid_mi355x_free_batch(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.free()
+
+    def buffers(self, i):
+        """(pab, hab, forces, virial) device views of list i."""
+        p, h, f, v = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        n = C.c_long()
+        lib().grid_mi355x_batch_buffers(self.h, C.c_int(i), C.byref(p), C.byref(h), C.byref(n),
+                                        C.byref(f), C.byref(v))
+        a = self.lists[i].args
+        return (_DevView(p.value, (n.value,)), _DevView(h.value, (n.value,)),
+                _DevView(f.value, (a.natoms, 3)), _DevView(v.value, (3, 3)))
+
+    def grid(self, i, level, potential=False):
+        shape = tuple(int(x) for x in self.lists[i].args.grid_shapes()[level])
+        ptr = lib().grid_mi355x_batch_grid(self.h, C.c_int(i), C.c_int(level),
+                                           C.c_int(1 if potential else 0))
+        return _DevView(ptr, shape)
+
+    def set_inputs(self, i, pab=None, V=None):
+        """Uploads list i's density blocks (its buffer_size values) and/or
+        potential grids."""
+        if pab is not None:
+            buf = self.buffers(i)[0]
+            full = np.zeros(buf.size)
+            n = min(len(pab), buf.size)  # the blocks the list addresses
+            full[:n] = pab[:n]
+            buf.upload(full)
+        if V is not None:
+            for l, v in enumerate(V):
+                self.grid(i, l, potential=True).upload(v)
+
+    def run(self, what=3, func=GRID_FUNC_AB, compute_tau=False, forces=False):
+        lib().grid_mi355x_batch_run(self.h, C.c_int(what), C.c_int(func), C.c_bool(compute_tau),
+                                    C.c_bool(forces))
+
+    def outputs(self, i):
+        """(rho levels, hab, forces, virial) of list i, on the host."""
+        a = self.lists[i].args
+        pab, hab, f, v = self.buffers(i)
+        rho = [self.grid(i, l).numpy() for l in range(a.nlevels)]
+        h = np.zeros(a.buffer_size)
+        hv = hab.numpy()
+        n = min(len(hv), a.buffer_size)
+        h[:n] = hv[:n]
+        return rho, h, f.numpy(), v.numpy()
+
+    def task_points(self):
+        n = C.c_longlong()
+        lib().grid_mi355x_batch_task_points(self.h, C.byref(n))
+        return int(n.value)

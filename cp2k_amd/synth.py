@@ -508,6 +508,33 @@ def config(name, seed=0, **kw):
     return args, pab, V
 
 
+def batch_config(nlists=256, seed=0, **kw):
+    """Independent task lists of uneven size for batching: per list 8-128
+    atoms (uniform) in a cubic cell of 6-14 A, one grid level of 32^3-112^3
+    points (growing with the cell), primitive shells l <= 2 with exponents
+    log-uniform in 0.1-20 bohr^-2.  Returns [(TaskListArgs, pab, [V])]."""
+    rng = np.random.default_rng(seed)
+    lo_atoms, hi_atoms = kw.get("atoms", (8, 128))
+    out = []
+    for i in range(nlists):
+        natoms = int(rng.integers(lo_atoms, hi_atoms + 1))
+        LA = float(rng.uniform(6.0, 14.0))
+        L = LA * ANGSTROM
+        cell = cubic_cell(L)
+        n = _even_smooth(32 + (LA - 6.0) / 8.0 * 80.0)
+        npts, dh, dh_inv, cutoff = grid_levels(cell, n, 1)
+        bases = [make_basis(rng, [(0, 0, 1, 1), (1, 1, 1, 1), (2, 2, 1, 1)], 0.1, 20.0,
+                            primitive=True) for _ in range(2)]
+        pos = rng.uniform(0, L, (natoms, 3))
+        kinds = (np.arange(natoms) % 2) + 1
+        args = build_task_list(cell, True, pos, kinds, bases, npts, dh, dh_inv, cutoff,
+                               1.0e9, kw.get("eps", 1.0e-10), rng,
+                               flip_fraction=kw.get("flip", 0.3),
+                               meta=dict(config=f"batch{nlists}[{i}]"))
+        out.append((args, random_density(args, rng), random_potential(args, rng)))
+    return out
+
+
 def _even_smooth(n):
     n = max(16, int(round(n)))
     while not (n % 2 == 0 and _smooth(n)):

@@ -105,7 +105,7 @@ void grid_mi355x_synchronize(void);
  * tiles, 8 collocation straight into HBM, 9 integrate points, 10 cxyz ->
  * hab/forces pass, 11 per-task kernels (lp > 6), 12 rounding of the
  * potential to float (FP32-mixed integrate), 13 general-cell point kernels
- * (collocate or integrate). */
+ * (collocate or integrate), 14 batch graph replays. */
 void grid_mi355x_set_timing(const bool enable);
 void grid_mi355x_reset_timing(void);
 /* Returns total milliseconds and number of launches of kernel id. */
@@ -122,6 +122,32 @@ void grid_mi355x_task_points_by_lp(const grid_task_list *task_list, long long *p
 /* Number of tasks that were assigned to the orthorhombic / general kernels. */
 void grid_mi355x_task_counts(const grid_task_list *task_list, long *ortho,
                              long *general, long *skipped);
+
+/* Batched task lists (many independent lists in one set of launches).
+ * create: the lists (each from grid_create_task_list; they stay owned by the
+ * caller and may be freed afterwards) are merged into one list whose every
+ * launch covers all of them; the batch owns HBM buffers for each list:
+ * pab/hab (laid out as that list's blocks, length doubles), forces [natoms][3],
+ * virial [3][3], and per level a density and a potential grid.
+ * run: what = 1 collocate (density from pab), 2 integrate (hab, and with
+ * forces also forces and virial, from the potential), 3 both; each distinct
+ * (what, func, compute_tau, forces, precision) is captured once into a
+ * hipGraph and replayed.  Work is enqueued on the library stream; the call
+ * returns without waiting (grid_mi355x_synchronize). */
+typedef struct grid_mi355x_batch grid_mi355x_batch;
+void grid_mi355x_create_batch(const int nlists, const grid_task_list *const *lists,
+                              grid_mi355x_batch **batch);
+void grid_mi355x_free_batch(grid_mi355x_batch *batch);
+void grid_mi355x_batch_buffers(const grid_mi355x_batch *batch, const int list,
+                               double **pab, double **hab, long *length,
+                               double **forces, double **virial);
+double *grid_mi355x_batch_grid(const grid_mi355x_batch *batch, const int list,
+                               const int level, const int potential);
+void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what,
+                           const enum grid_func func, const bool compute_tau,
+                           const bool forces);
+/* Task points of all lists of the batch. */
+void grid_mi355x_batch_task_points(const grid_mi355x_batch *batch, long long *points);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,112 @@
+"""Batched task lists (grid_mi355x_create_batch / grid_mi355x_batch_run).
+
+A batch of lists with different cells, grid sizes and level counts (cubic
+one-level lists, a multi-level water box, a triclinic list) must give each
+list exactly what the list gives on its own (summation order aside, 1e-12
+as in the FP64 parity tests).  A run replays a captured hipGraph: every
+replay is checked, with eager calls and synchronisations in between.
+"""
+import pytest
+
+import replay
+from cp2k_amd import grid, precision, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+FTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def lists():
+    from cp2k_amd import build
+    if not build.LIB.exists():
+        build.build()
+    grid.set_precision(grid.FP64)
+    out = synth.batch_config(4, seed=5, atoms=(8, 20))
+    out.append(synth.config("h2o2", seed=7, n=32))
+    out.append(synth.config("tri10", seed=8))
+    return out
+
+
+def tol_of(i, lists):
+    # the wrap-stress triclinic list sums thousands of images per point with
+    # cancellation: 1e-11, as in the FP64 parity tests
+    return 1e-11 if lists[i][0].meta.get("config") == "tri10" else TOL
+
+
+def eager(args, pab, V):
+    gt = grid.GridTaskList(args)
+    rho = gt.collocate(100, pab)
+    hab, f, v = gt.integrate(V, pab, forces=True, virial=True)
+    return rho, hab, f, v
+
+
+def check(got, want, tag, tol=TOL):
+    rho, hab, f, v = got
+    rho0, hab0, f0, v0 = want
+    for a, b in zip(rho, rho0):
+        assert replay.rel_err(a, b) < tol, tag
+    assert replay.rel_err(hab, hab0) < tol, tag
+    assert replay.rel_err(f, f0) < FTOL, tag
+    assert replay.rel_err(v, v0) < FTOL, tag
+
+
+def test_batch_matches_single_lists(lists):
+    want = [eager(a, p, V) for a, p, V in lists]
+    gts = [grid.GridTaskList(a) for a, _, _ in lists]
+    batch = grid.GridBatch(gts)
+    for i, (a, p, V) in enumerate(lists):
+        batch.set_inputs(i, pab=p, V=V)
+    assert batch.task_points() == sum(g.task_points() for g in gts)
+    for rep in range(3):
+        print("batch replay", rep, flush=True)
+        batch.run(3, forces=True)
+        grid.synchronize()
+        for i in range(len(lists)):
+            check(batch.outputs(i), want[i], (rep, i), tol_of(i, lists))
+        # eager work between replays, on the same stream
+        a, p, V = lists[rep]
+        check(eager(a, p, V), want[rep], ("eager", rep), tol_of(rep, lists))
+        grid.synchronize()
+
+
+def test_batch_first_list_against_oracle(lists):
+    a, p, V = lists[0]
+    batch = grid.GridBatch([grid.GridTaskList(x) for x, _, _ in lists[:2]])
+    batch.set_inputs(0, pab=p, V=V)
+    batch.set_inputs(1, pab=lists[1][1], V=lists[1][2])
+    batch.run(1)
+    batch.run(2, forces=True)
+    grid.synchronize()
+    rho, hab, f, v = batch.outputs(0)
+    ot = O.TaskList(a)
+    for g, w in zip(rho, ot.collocate(100, p)):
+        assert replay.rel_err(g, w) < TOL
+    h2, f2, v2 = ot.integrate(V, p, forces=True, virial=True)
+    assert replay.rel_err(hab, h2) < TOL
+    assert replay.rel_err(f, f2) < FTOL
+    assert replay.rel_err(v, v2) < FTOL
+
+
+def test_batch_fp32_mixed(lists):
+    want = [eager(a, p, V) for a, p, V in lists]
+    batch = grid.GridBatch([grid.GridTaskList(a) for a, _, _ in lists])
+    for i, (a, p, V) in enumerate(lists):
+        batch.set_inputs(i, pab=p, V=V)
+    grid.set_precision(grid.FP32_MIXED)
+    try:
+        for rep in range(2):
+            batch.run(3, forces=True)
+            grid.synchronize()
+            for i in range(len(lists)):
+                rho, hab, f, v = batch.outputs(i)
+                r = precision.report(rho=(rho, want[i][0]), hab=(hab, want[i][1]),
+                                     forces=(f, want[i][2]))
+                print("batch fp32", rep, i, {k: x for k, x in r.items() if k != "rho_levels"},
+                      flush=True)
+                assert r["rho_linf"] <= precision.RHO_LINF_BAR, (rep, i, r)
+                assert r["forces_linf"] <= precision.FORCE_BAR, (rep, i, r)
+    finally:
+        grid.set_precision(grid.FP64)
