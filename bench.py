@@ -50,9 +50,21 @@ HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 
 # kernels of one collocate / integrate call, by rocprofv3 kernel name
 CALL_KERNELS = {
-    "collocate": ("cxyz_wave_kernel", "collocate_tile_kernel", "collocate_wave_kernel",
-                  "collocate_kernel"),
-    "integrate": ("integrate_wave_kernel", "finish_wave_kernel", "integrate_kernel"),
+    "collocate": ("cxyz_lane_kernel", "cxyz_wave_kernel", "collocate_tile_kernel",
+                  "collocate_wave_kernel", "general_collocate_kernel", "collocate_kernel"),
+    "integrate": ("integrate_wave_kernel", "finish_wave_kernel", "general_integrate_kernel",
+                  "to_float_kernel", "integrate_kernel"),
+}
+# timing ids (grid.KERNEL_NAMES) inside each call
+CALL_IDS = {
+    "collocate": ("cxyz", "collocate_tiles", "collocate_direct", "per_task", "general_points"),
+    "integrate": ("integrate_points", "finish", "per_task", "general_points", "to_float"),
+}
+WORKLOADS = {
+    "h2o64": "64 H2O, 12.4 A cube, 4 levels 128/72/42/24, DZVP-like l<=2",
+    "tri1024": "1024 atoms, triclinic 80-100 deg, 4 levels 200/112/64/36, TZV2P-like l<=3",
+    "hl2048": "2048 atoms, orthorhombic, 5 levels 256/150/84/48/28, l<=4",
+    "batch256": "256 lists of 8-128 atoms, cubic 6-14 A, one level 32^3-112^3, l<=2",
 }
 
 
@@ -67,6 +79,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-fp32", action="store_true",
                    help="skip the FP32-mixed leg (throughput and per-result errors)")
+    p.add_argument("--secondary", default="tri1024",
+                   help="comma-separated workloads also measured (FP64 and FP32-mixed, "
+                        "no CPU leg) and reported under 'secondary'; empty for none")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="before the steps, run 3 device-to-device copies of MIB MiB: "
@@ -235,6 +250,11 @@ def cpu_baseline(args, pab, V, seconds):
                        f"collocate {t1 - t0:.2f} s, integrate {t2 - t1:.2f} s")}
 
 
+def log(msg):
+    """Progress of the run, on stderr (the JSON line goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def timed_steps(step, steps):
     from cp2k_amd import grid
     grid.synchronize()
@@ -304,14 +324,61 @@ def main():
         print(f"bench.py measures one GPU only (this backend has no multi-GPU path); "
               f"got --gpus {opt.gpus}", file=sys.stderr)
         sys.exit(2)
-    from cp2k_amd import grid, synth
+    res = measure(opt, opt.config, cpu=not opt.no_cpu)
+    if opt.secondary and not opt.config.startswith("batch"):
+        res["secondary"] = {}
+        for name in [x for x in opt.secondary.split(",") if x and x != opt.config]:
+            r = measure(opt, name, cpu=False)
+            res["secondary"][name] = {k: r[k] for k in (
+                "value", "unit", "ms_per_step", "dtype", "config", "kernel_ms_per_step",
+                "roofline", "fp32_mixed") if k in r}
+    print(json.dumps(res))
 
+
+def roofline(opt, kt, by_lp, abytes_fn, traffic_fn):
+    """Roofline object of the dominant call from the per-kernel timings."""
+    from cp2k_amd import grid
+    dom = max(("collocate", "integrate"), key=lambda k: kt[k][0])
+    ms, n = kt[dom]
+    avg_ms = ms / max(n, 1)
+    shift = 0 if (dom == "collocate" or not opt.forces) else 2
+    flops = sum(2.0 * (lp + shift + 1) * p for lp, p in enumerate(by_lp))
+    tflops = flops / (avg_ms * 1e-3) / 1e12
+    abytes = abytes_fn(dom)
+    achieved = abytes / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_fn(dom)
+    names = [k for k in grid.KERNEL_NAMES if k in CALL_IDS[dom] and kt[k][0] > 0]
+    return {
+        "bound": "hbm",
+        "kernel": f"{dom} call ({', '.join(names)})",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": abytes,
+        "avg_launch_ms": round(avg_ms, 4),
+        "measured_hbm_gbs": (round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None),
+        "fp64_tflops": round(tflops, 3),
+        "fp64_frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+        "algorithmic_flops_per_launch": flops,
+    }
+
+
+def measure(opt, name, cpu):
+    """One workload: FP64 (or --precision) steps, the FP32-mixed leg, the
+    roofline of the dominant call and, with cpu, the CPU baseline."""
+    from cp2k_amd import grid, synth
+    if name.startswith("batch"):
+        return measure_batch(opt, name)
     grid.set_precision(grid.FP64 if opt.precision == "fp64" else grid.FP32_MIXED)
     t0 = time.time()
-    args, pab, V = synth.config(opt.config, seed=0)
+    args, pab, V = synth.config(name, seed=0)
     gen_s = time.time() - t0
+    log(f"{name}: generated {args.ntasks} tasks in {gen_s:.1f} s")
     gt = grid.GridTaskList(args)
     points = gt.task_points()
+    log(f"{name}: task list built, {points} task points")
     by_lp = gt.task_points_by_lp()
     pab_d = grid.DeviceArray.from_numpy(pab)
     rho_d = [grid.DeviceArray(s) for s in args.grid_shapes()]
@@ -338,27 +405,10 @@ def main():
     grid.synchronize()
     grid.reset_timing()
     grid.set_timing(True)
-    grid.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(opt.steps):
-        step()
-    grid.synchronize()
-    t1 = time.perf_counter()
+    ms_step = timed_steps(step, opt.steps)
     grid.set_timing(False)
-    ms_step = (t1 - t0) * 1e3 / opt.steps
     kt = {k: grid.kernel_time_ms(i) for i, k in enumerate(grid.KERNEL_NAMES)}
     value = 2.0 * points / (ms_step * 1e-3) / 1e9
-    # dominant call and its FP64 roofline; lp shift of the call: collocate
-    # GRID_FUNC_AB 0, integrate hab 0 (forces 1, virial 2 on la/lb)
-    dom = max(("collocate", "integrate"), key=lambda k: kt[k][0])
-    ms, n = kt[dom]
-    avg_ms = ms / max(n, 1)
-    shift = 0 if (dom == "collocate" or not opt.forces) else 2
-    flops = sum(2.0 * (lp + shift + 1) * p for lp, p in enumerate(by_lp))
-    tflops = flops / (avg_ms * 1e-3) / 1e12
-    abytes = algorithmic_bytes(args, points, dom)
-    achieved = abytes / (avg_ms * 1e-3) / 1e9
-    traffic = profile_traffic(dom, opt.config) if not opt.forces else None
     out = {
         "metric": "collocate+integrate Gpts/s & HBM GB/s; FP32-mixed vs FP64 at <=2e-6 rel rho err",
         "value": round(value, 3),
@@ -373,44 +423,137 @@ def main():
         "dtype": "f64" if opt.precision == "fp64" else "f32-mixed(f64-accumulate)",
         "data": "synthetic (seeded cp2k_amd.synth, random density/potential)",
         "config": {
-            "workload": {"h2o64": "64 H2O, 12.4 A cube, 4 levels 128/72/42/24, DZVP-like l<=2",
-                         }.get(opt.config, opt.config),
-            "config_name": opt.config,
+            "workload": WORKLOADS.get(name, name),
+            "config_name": name,
             "ntasks": args.ntasks,
             "task_points": points,
             "task_points_by_lp": by_lp,
             "levels": [int(n[0]) for n in args.npts_local],
             "forces": bool(opt.forces),
             "tasks_per_s": round(2 * args.ntasks / (ms_step * 1e-3), 1),
+            "generate_s": round(gen_s, 2),
         },
         "kernel_ms_per_step": {k: round(v[0] / max(opt.steps, 1), 4) for k, v in kt.items()},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"{dom} call (" + ", ".join(
-                k for k in grid.KERNEL_NAMES[6:] if kt[k][0] > 0 and
-                ((dom == "collocate") == (k in ("cxyz", "collocate_tiles", "collocate_direct",
-                                                 "per_task")) or k == "per_task")) + ")",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": abytes,
-            "avg_launch_ms": round(avg_ms, 4),
-            "measured_hbm_gbs": (round(traffic / (avg_ms * 1e-3) / 1e9, 1)
-                                 if traffic else None),
-            "fp64_tflops": round(tflops, 3),
-            "fp64_frac": round(tflops / FP64_PEAK_TFLOPS, 4),
-            "algorithmic_flops_per_launch": flops,
-        },
+        "roofline": roofline(opt, kt, by_lp, lambda dom: algorithmic_bytes(args, points, dom),
+                             lambda dom: (profile_traffic(dom, name) if not opt.forces else None)),
         "cpu_baseline": None,
     }
+    log(f"{name}: {opt.steps} steps of {ms_step:.3f} ms")
     if not opt.no_fp32:
         out["fp32_mixed"] = fp32_leg(opt, gt, step, pab_d, rho_d, V_d, points, ms_step)
-    if not opt.no_cpu:
+        log(f"{name}: FP32-mixed leg done")
+    if cpu:
         out["cpu_baseline"] = cpu_baseline(args, pab, V, opt.cpu_seconds)
-    out["config"]["generate_s"] = round(gen_s, 2)
-    print(json.dumps(out))
+    for a in [pab_d, hab_d, f_d, v_d] + rho_d + V_d:
+        if a is not None:
+            a.free()
+    gt.free()
+    return out
+
+
+def measure_batch(opt, name):
+    """Batched lists: one step = one replay of the batch graph (collocate
+    then integrate of every list).  Also timed: the same lists called one by
+    one through the device entry points (unbatched)."""
+    from cp2k_amd import grid, precision, synth
+    nlists = int(name[5:] or 256)
+    t0 = time.time()
+    lists = synth.batch_config(nlists, seed=0)
+    gen_s = time.time() - t0
+    log(f"{name}: generated {nlists} lists in {gen_s:.1f} s")
+    grid.set_precision(grid.FP64)
+    gts = []
+    for i, (a, _, _) in enumerate(lists):
+        gts.append(grid.GridTaskList(a))
+        if i % 32 == 31:
+            log(f"{name}: {i + 1} task lists built")
+    batch = grid.GridBatch(gts)
+    log(f"{name}: batch built")
+    for i, (a, p, V) in enumerate(lists):
+        batch.set_inputs(i, pab=p, V=V)
+    points = batch.task_points()
+    by_lp = [0] * 11
+    for g in gts:
+        for lp, x in enumerate(g.task_points_by_lp()):
+            by_lp[lp] += x
+    ntasks = sum(a.ntasks for a, _, _ in lists)
+
+    def step():
+        batch.run(3, forces=opt.forces)
+
+    def timed(mode):
+        grid.set_precision(mode)
+        for _ in range(max(opt.warmup, 1)):
+            step()
+        grid.synchronize()
+        grid.reset_timing()
+        grid.set_timing(True)
+        ms = timed_steps(step, opt.steps)
+        grid.set_timing(False)
+        return ms
+
+    ms_step = timed(grid.FP64)
+    log(f"{name}: batch steps of {ms_step:.3f} ms")
+    kt = {k: grid.kernel_time_ms(i) for i, k in enumerate(grid.KERNEL_NAMES)}
+    ref = [batch.outputs(i) for i in range(nlists)]
+    out = {
+        "metric": "collocate+integrate Gpts/s & HBM GB/s; FP32-mixed vs FP64 at <=2e-6 rel rho err",
+        "value": round(2.0 * points / (ms_step * 1e-3) / 1e9, 3),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": opt.steps,
+        "warmup": opt.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cp2k_amd.synth.batch_config, random density/potential)",
+        "config": {"workload": WORKLOADS.get(name, name), "config_name": name, "lists": nlists,
+                   "ntasks": ntasks, "task_points": points, "task_points_by_lp": by_lp,
+                   "tasks_per_s": round(2 * ntasks / (ms_step * 1e-3), 1),
+                   "forces": bool(opt.forces), "generate_s": round(gen_s, 2)},
+        "kernel_ms_per_step": {k: round(v[0] / max(opt.steps, 1), 4) for k, v in kt.items()},
+        "cpu_baseline": None,
+    }
+    # the same work list by list, unbatched (device entry points, no graph)
+    bufs = []
+    for g, (a, p, V) in zip(gts, lists):
+        bufs.append((grid.DeviceArray.from_numpy(p), [grid.DeviceArray(s) for s in a.grid_shapes()],
+                     [grid.DeviceArray.from_numpy(v) for v in V], grid.DeviceArray(a.buffer_size)))
+
+    def unbatched():
+        for g, (p_d, r_d, v_d, h_d) in zip(gts, bufs):
+            g.collocate_dev(100, p_d, r_d)
+            g.integrate_dev(v_d, h_d)
+
+    unbatched()
+    ms_unb = timed_steps(unbatched, max(1, opt.steps // 2))
+    log(f"{name}: unbatched steps of {ms_unb:.3f} ms")
+    out["unbatched_ms_per_step"] = round(ms_unb, 3)
+    out["batch_speedup"] = round(ms_unb / ms_step, 3)
+    for p_d, r_d, v_d, h_d in bufs:
+        for x in [p_d, h_d] + r_d + v_d:
+            x.free()
+    if not opt.no_fp32:
+        ms32 = timed(grid.FP32_MIXED)
+        got = [batch.outputs(i) for i in range(nlists)]
+        grid.set_precision(grid.FP64)
+        worst = {}
+        for g_, r_ in zip(got, ref):
+            e = precision.report(rho=(g_[0], r_[0]), hab=(g_[1], r_[1]))
+            for k in ("rho_linf", "rho_l2", "hab_linf"):
+                worst[k] = max(worst.get(k, 0.0), e[k])
+        out["fp32_mixed"] = {
+            "value": round(2.0 * points / (ms32 * 1e-3) / 1e9, 3),
+            "unit": "Gpts/s",
+            "ms_per_step": round(ms32, 3),
+            "speedup_vs_fp64": round(ms_step / ms32, 3),
+            "errors_vs_fp64_worst_list": {k: round(v, 12) for k, v in worst.items()},
+            "within_bars": bool(worst["rho_linf"] <= precision.RHO_LINF_BAR),
+        }
+    batch.free()
+    return out
 
 
 if __name__ == "__main__":

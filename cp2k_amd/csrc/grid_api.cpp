@@ -328,13 +328,18 @@ struct TileInfo {
 struct TilePlan {
   DevBuf d;
   std::vector<std::array<std::pair<int, int>, 2>> range;  // [level][bucket] = (first, count)
+  std::pair<int, int> mrange[2];                          // [bucket]: all levels (merged)
 };
 
 // Launch class of a task in merged mode.
 enum TaskClass : signed char { kClsOrthoClassed = 0, kClsOrthoOther = 1, kClsGeneral = 2, kClsSkip = 3 };
 
 // Merged launches for one lp shift: task index lists of the launch kinds.
-enum MergedList { kMSetup, kMOrthoA, kMOrthoB, kMGenA, kMGenB, kMOrthoRest, kMGenRest, kMNum };
+enum MergedList {
+  kMSetup, kMOrthoA, kMOrthoB, kMGenA, kMGenB, kMOrthoRest, kMGenRest,
+  kMDirectA, kMDirectB,  // classed orthorhombic tasks of levels without collocation tiles
+  kMNum
+};
 struct MergedPlan {
   DevBuf d;
   int off[kMNum + 1] = {};  // list m = d[off[m] .. off[m + 1])
@@ -450,6 +455,8 @@ void upload(DevBuf &b, const std::vector<T> &v) {
   void *p = b.get(n ? n : sizeof(T));
   if (n) HIP_CHECK(hipMemcpy(p, v.data(), n, hipMemcpyHostToDevice));
 }
+
+int lds_limit();
 
 void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevels,
                      int natoms, int nkinds, int nblocks, const int *block_offsets,
@@ -914,15 +921,19 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     if (o.end <= o.begin || !o.classed) continue;
     const Layout &y = tl->layouts[l];
     const long long nx = y.npts_local[0], ny = y.npts_local[1], nz = y.npts_local[2];
-    if (nx * ny * nz <= kTileDoubles) {
+    // the tile shares LDS with at least four wave regions of the largest
+    // tables (lp <= 6, this level's cube extent)
+    const long long wave_max = 8LL * (84 + 3LL * o.max_ext * 7) + 16;
+    const long long cap = std::min(kTileDoubles, (lds_limit() - 4 * wave_max) / 8);
+    if (nx * ny * nz <= cap) {
       o.by = (int)ny;
       o.bz = (int)nz;
-    } else if (kTileDoubles / (nx * nz) >= 4) {
-      o.by = (int)(kTileDoubles / (nx * nz));
+    } else if (cap / (nx * nz) >= 4) {
+      o.by = (int)(cap / (nx * nz));
       o.bz = (int)nz;
-    } else if (nx * 8 * 2 <= kTileDoubles) {
+    } else if (nx * 8 * 2 <= cap) {
       o.by = (int)std::min<long long>(8, ny);
-      o.bz = (int)std::min<long long>(nz, kTileDoubles / (nx * o.by));
+      o.bz = (int)std::min<long long>(nz, cap / (nx * o.by));
     } else {
       continue;  // rows too long for a tile: straight into HBM
     }
@@ -1041,7 +1052,6 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   upload(tl->d_geo_kmin, geo_kmin);
   upload(tl->d_geo_rows, geo_rows);
   upload(tl->d_items, items);
-  if (nlevels > 64) fatal("at most 64 grid levels are supported");
   tl->d_cart.get(sizeof(double) * std::max<long long>(tl->cart_len, 1));
   tl->d_hab_cart.get(sizeof(double) * std::max<long long>(tl->cart_len, 1));
   tl->d_task_out.get(sizeof(double) * kTaskOutStride * (size_t)ntasks);
@@ -1160,7 +1170,14 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
   w.wave_bytes = (8 * nd + 15) & ~15;
   const int limit = lds_limit();
   if (kind == kPlanWindow) {
-    const long long npts = (long long)tl->npts_local[3 * level] * o.by * o.bz;
+    long long npts = 0;  // the largest tile of the launch
+    if (level >= 0) {
+      npts = (long long)tl->npts_local[3 * level] * o.by * o.bz;
+    } else {
+      for (int l = 0; l < tl->nlevels; l++)
+        if (tl->olev[l].tiled)
+          npts = std::max(npts, (long long)tl->npts_local[3 * l] * tl->olev[l].by * tl->olev[l].bz);
+    }
     const int most = lp <= 4 ? 12 : 8;  // launch bounds of the tile kernels
     for (int nw : {12, 10, 8, 6, 4})
       if (nw <= most && 8 * npts + (long long)nw * w.wave_bytes <= limit) {
@@ -1171,6 +1188,7 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
         break;
       }
     if (!w.window) fatal("level %d: no room for the collocation tile in LDS", level + 1);
+    if (level < 0) w.by = w.bz = 0;  // per unit
   }
   if (!w.window) {
     w.nwaves = 1;
@@ -1219,12 +1237,23 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh) {
           const int a = t.start[v0], b = t.start[v1];
           if (b <= a) continue;
           const int nw = (int)std::max<long long>(1, ((long long)(b - a) + per / 2) / per);
-          for (int w = 0; w < nw; w++) units.push_back(TileUnit{a, b, w, nw, t.gy0, t.gz0});
+          for (int w = 0; w < nw; w++)
+            units.push_back(TileUnit{a, b, w, nw, t.gy0, t.gz0, l, tl->olev[l].by, tl->olev[l].bz});
         }
       }
       plan.range[l][r] = std::make_pair(first, (int)units.size() - first);
     }
   }
+  // merged launches: the units of every level, bucket by bucket
+  const int nbase = (int)units.size();
+  for (int r = 0; r < 2; r++) {
+    const int first = (int)units.size();
+    for (int l = 0; l < tl->nlevels; l++)
+      for (int u = 0; u < plan.range[l][r].second; u++)
+        units.push_back(units[plan.range[l][r].first + u]);
+    plan.mrange[r] = std::make_pair(first, (int)units.size() - first);
+  }
+  (void)nbase;
   upload(plan.d, units);
   return plan;
 }
@@ -1390,6 +1419,7 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
     } else if (c == kClsOrthoClassed && wave && lp <= 6) {
       lists[kMSetup].push_back(t);
       lists[lp <= 4 ? kMOrthoA : kMOrthoB].push_back(t);
+      if (!tl->olev[tl->task_level[t]].tiled) lists[lp <= 4 ? kMDirectA : kMDirectB].push_back(t);
     } else {
       lists[kMOrthoRest].push_back(t);
     }
@@ -1431,14 +1461,57 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
       HIP_CHECK(launch_cxyz_wave(Q, s));
     }
-    for (int m : {kMOrthoA, kMOrthoB}) {
-      PointParams R = sub(m);
-      if (R.ntasks <= 0) continue;
-      const int lpb = m == kMOrthoA ? 4 : 6;
-      R.wp = make_wave_plan(tl, -1, std::min(lpb, mp.lp[m]), P.ld, kPlanPoints);
-      TimedLaunch tm(mode == kIntegrate ? 9 : 8, s);
-      HIP_CHECK(mode == kIntegrate ? launch_integrate_wave(R, lpb, s)
-                                   : launch_collocate_wave(R, lpb, s));
+    if (mode == kIntegrate) {
+      for (int m : {kMOrthoA, kMOrthoB}) {
+        PointParams R = sub(m);
+        if (R.ntasks <= 0) continue;
+        const int lpb = m == kMOrthoA ? 4 : 6;
+        R.wp = make_wave_plan(tl, -1, std::min(lpb, mp.lp[m]), P.ld, kPlanPoints);
+        TimedLaunch tm(9, s);
+        HIP_CHECK(launch_integrate_wave(R, lpb, s));
+      }
+    } else {
+      // tiled levels through their LDS tiles (one launch per bucket over all
+      // levels), the others straight into HBM
+      const TilePlan &tp = tile_plan(tl, sh);
+      long long max_window = 0;
+      for (int l = 0; l < tl->nlevels; l++)
+        if (tl->olev[l].tiled)
+          max_window = std::max(max_window, (long long)tl->npts_local[3 * l] * tl->olev[l].by *
+                                                tl->olev[l].bz);
+      for (int r = 0; r < 2; r++) {
+        if (tp.mrange[r].second <= 0) continue;
+        const int lpb = r == 0 ? 4 : 6;
+        const int lp = std::min(lpb, mp.lp[r == 0 ? kMOrthoA : kMOrthoB]);
+        PointParams R = P;
+        R.units = (const TileUnit *)tp.d.p;
+        const int np = lp + 1;
+        const long long wave = 8LL * ((lp + 1) * (lp + 2) * (lp + 3) / 6 + 3LL * tl->max_ext_all * np) + 16;
+        if (8 * max_window + 4 * wave <= lds_limit()) {
+          R.unit_base = tp.mrange[r].first;
+          R.wp = make_wave_plan(tl, -1, lp, P.ld, kPlanWindow);
+          TimedLaunch tm(7, s);
+          HIP_CHECK(launch_collocate_tiles(R, lpb, tp.mrange[r].second, s));
+        } else {
+          // the largest tile and the largest tables do not fit together:
+          // one launch per level
+          for (int l = 0; l < tl->nlevels; l++) {
+            if (!tl->olev[l].tiled || tp.range[l][r].second <= 0) continue;
+            R.unit_base = tp.range[l][r].first;
+            R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanWindow);
+            TimedLaunch tm(7, s);
+            HIP_CHECK(launch_collocate_tiles(R, lpb, tp.range[l][r].second, s));
+          }
+        }
+      }
+      for (int m : {kMDirectA, kMDirectB}) {
+        PointParams R = sub(m);
+        if (R.ntasks <= 0) continue;
+        const int lpb = m == kMDirectA ? 4 : 6;
+        R.wp = make_wave_plan(tl, -1, std::min(lpb, mp.lp[m]), P.ld, kPlanPoints);
+        TimedLaunch tm(8, s);
+        HIP_CHECK(launch_collocate_wave(R, lpb, s));
+      }
     }
     for (int m : {kMGenA, kMGenB}) {
       PointParams R = sub(m);
@@ -1519,12 +1592,12 @@ const Level *set_levels(TaskListImpl *tl, double *const *grids, hipStream_t s,
   TaskListImpl::LevelSlot &ls = tl->level_slots[tl->next_level_slot];
   tl->next_level_slot = (tl->next_level_slot + 1) % TaskListImpl::kLevelSlots;
   if (!ls.h) {
-    HIP_CHECK(hipHostMalloc((void **)&ls.h, sizeof(Level) * 64));
+    HIP_CHECK(hipHostMalloc((void **)&ls.h, sizeof(Level) * (size_t)std::max(tl->nlevels, 1)));
     HIP_CHECK(hipEventCreateWithFlags(&ls.copied, hipEventDisableTiming));
   } else {
     HIP_CHECK(hipEventSynchronize(ls.copied));  // the slot's previous copy is done
   }
-  ls.d.get(sizeof(Level) * 64);
+  ls.d.get(sizeof(Level) * (size_t)std::max(tl->nlevels, 1));
   long long off = 0;
   for (int l = 0; l < tl->nlevels; l++) {
     Level &L = ls.h[l];

@@ -79,6 +79,7 @@ struct TileItem {
 struct TileUnit {
   int ibeg, iend, first, stride;
   int gy0, gz0;        // tile origin in the local grid
+  int level, by, bz;   // the tile's level and its rows and planes (nx * by * bz points)
 };
 
 // One (matrix block, iset, jset) sub-block: the unit of (de)contraction.

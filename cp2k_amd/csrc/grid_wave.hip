@@ -218,12 +218,12 @@ template <int LP, int NCL, typename R>
 __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, const GeoClass &G,
                                            const Layout &lay, const WavePlan &wp,
                                            const R *tab, const R *cx,
-                                           const TileItem &I, int gy0, int gz0, const int cw[3],
+                                           const TileItem &I, const TileUnit &U, const int cw[3],
                                            int c0, int c_end, double *win, int lane) {
   const int NP = wp.np;
   const R *tabz = tab + 2 * wp.ext * NP;
   const int nx = lay.npts_local[0];
-  const int nxby = nx * wp.by;
+  const int nxby = nx * U.by;
   const int ngz = lay.npts_global[2];
   Column C[NCL];
   int lbase[NCL];
@@ -236,9 +236,9 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
     C[u] = load_column(P, G, lay, cw, c < c_end ? c : G.ncol);
     lbase[u] = 0;
     if (C[u].kl <= 0) {
-      const int ly = C[u].base / nx - gy0;  // base = y * nx + x
-      if ((unsigned)ly < (unsigned)wp.by)
-        lbase[u] = C[u].base - gy0 * nx;
+      const int ly = C[u].base / nx - U.gy0;  // base = y * nx + x
+      if ((unsigned)ly < (unsigned)U.by)
+        lbase[u] = C[u].base - U.gy0 * nx;
       else
         C[u].kl = 1;  // outside the tile: never for items built on the host
     }
@@ -254,8 +254,8 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
 #pragma unroll 2
   for (int k = ka; k <= kb; k++) {
     const R *pz = tabz + (k - T.lo[2]) * NP;
-    const int tz = gz - gz0;
-    if ((unsigned)tz < (unsigned)wp.bz) {
+    const int tz = gz - U.gz0;
+    if ((unsigned)tz < (unsigned)U.bz) {
 #pragma unroll
       for (int u = 0; u < NCL; u++) {
         R v = 0;
@@ -269,13 +269,13 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
 }
 
 // Collocate the rows j0..j1 and planes k0..k1 of a task into an LDS tile
-// win[(z - gz0) * nx * by + (y - gy0) * nx + x].
+// win[(z - gz0) * nx * by + (y - gy0) * nx + x] of unit U.
 template <int LP, typename R>
 __device__ __forceinline__ void tile_columns(const PointParams &P, const Task &T,
                                              const GeoClass &G, const Layout &lay,
                                              const WavePlan &wp, const R *tab,
-                                             const R *cx, const TileItem &I, int gy0,
-                                             int gz0, double *win, int lane) {
+                                             const R *cx, const TileItem &I,
+                                             const TileUnit &U, double *win, int lane) {
   // two columns per lane fit the registers for lp <= 4 (FP64) or lp <= 6 (FP32)
   constexpr int L2 = sizeof(R) == 4 ? 6 : 4;
   int cw[3];
@@ -284,10 +284,10 @@ __device__ __forceinline__ void tile_columns(const PointParams &P, const Task &T
   const int c_end = P.geo_rows[G.row_off + (I.j1 + 1 - G.lo[1])];
   for (int c0 = c_beg; c0 < c_end;) {
     if (LP <= L2 && c_end - c0 > 64) {
-      tile_group<LP, (LP <= L2 ? 2 : 1), R>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
+      tile_group<LP, (LP <= L2 ? 2 : 1), R>(P, T, G, lay, wp, tab, cx, I, U, cw, c0, c_end, win, lane);
       c0 += (LP <= L2 ? 128 : 64);
     } else {
-      tile_group<LP, 1, R>(P, T, G, lay, wp, tab, cx, I, gy0, gz0, cw, c0, c_end, win, lane);
+      tile_group<LP, 1, R>(P, T, G, lay, wp, tab, cx, I, U, cw, c0, c_end, win, lane);
       c0 += 64;
     }
   }
@@ -676,9 +676,10 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
   const TileUnit U = units[P.unit_base + blockIdx.x];
-  const Level &L = levels[wp.level];
+  const Level &L = levels[U.level];
   const Layout &lay = L.lay;
-  for (int e = threadIdx.x; e < wp.window; e += blockDim.x) lds[e] = 0.0;
+  const int window = lay.npts_local[0] * U.by * U.bz;  // <= wp.window
+  for (int e = threadIdx.x; e < window; e += blockDim.x) lds[e] = 0.0;
   __syncthreads();
   const WaveLds W = wave_lds(lds, wp, wave);
   R *tab = reinterpret_cast<R *>(W.u);
@@ -691,24 +692,24 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
     const R *cx = stage_cxyz<R>(P, I.task, lp, W, lane);
     wave_tables(T, lay, lp, wp, tab, lane);  // ends with wave_sync
     switch (lp) {
-      case 0: tile_columns<0>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 1: tile_columns<1>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 2: tile_columns<2>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 3: tile_columns<3>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 4: tile_columns<4>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 5: if (LPB >= 5) tile_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
-      case 6: if (LPB >= 6) tile_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, cx, I, U.gy0, U.gz0, lds, lane); break;
+      case 0: tile_columns<0>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 1: tile_columns<1>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 2: tile_columns<2>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 3: tile_columns<3>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 4: tile_columns<4>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 5: if (LPB >= 5) tile_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 6: if (LPB >= 6) tile_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
       default: break;  // the host routes larger lp to the per-task kernels
     }
     wave_sync();
   }
   __syncthreads();
   const int nx = lay.npts_local[0], ny = lay.npts_local[1], nz = lay.npts_local[2];
-  for (int e = threadIdx.x; e < wp.window; e += blockDim.x) {
+  for (int e = threadIdx.x; e < window; e += blockDim.x) {
     const double v = lds[e];
     if (v == 0.0) continue;
     const int x = e % nx, r = e / nx;
-    const int y = U.gy0 + r % wp.by, z = U.gz0 + r / wp.by;
+    const int y = U.gy0 + r % U.by, z = U.gz0 + r / U.by;
     if (y < ny && z < nz) atomicAdd(&L.grid[((long long)z * ny + y) * nx + x], v);
   }
 }
