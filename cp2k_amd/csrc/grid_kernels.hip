@@ -206,7 +206,7 @@ __device__ __forceinline__ XSlots x_slots(const Task &T, int tid) {
 // ORTHO: the launch covers only orthorhombic (true) or only general (false)
 // tasks, so each instantiation carries one code path.
 template <int NW, int LPB, bool ORTHO>
-__global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
+__device__ __forceinline__ void collocate_task(const PointParams &P) {
   extern __shared__ double lds[];
   __shared__ Task sT;
   __shared__ Level sL;
@@ -421,6 +421,18 @@ __global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
   }
   if (P.count_only && tid == 0)
     P.task_out[launch_task(P, blockIdx.x) * kTaskOutStride] = counted;
+}
+
+template <int NW, int LPB, bool ORTHO>
+__global__ __launch_bounds__(64 * NW) void collocate_kernel(PointParams P) {
+  collocate_task<NW, LPB, ORTHO>(P);
+}
+
+// The same walk with P.count_only set: task points per task (its own kernel
+// name keeps it apart from collocation in profiles).
+template <int NW, bool ORTHO>
+__global__ __launch_bounds__(64 * NW) void count_points_kernel(PointParams P) {
+  collocate_task<NW, 4, ORTHO>(P);
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +883,7 @@ static hipError_t launch_task_kernel(K kernel, const PointParams &P, int nthread
 
 template <bool ORTHO>
 static hipError_t launch_collocate_kind(const PointParams &P, int lp, size_t lds, hipStream_t s) {
+  if (P.count_only) return launch_task_kernel(count_points_kernel<2, ORTHO>, P, 128, lds, s);
   switch (lpb_bucket(lp)) {
     case 4: return launch_task_kernel(collocate_kernel<2, 4, ORTHO>, P, 128, lds, s);
     case 8: return launch_task_kernel(collocate_kernel<2, 8, ORTHO>, P, 128, lds, s);
