@@ -52,6 +52,7 @@ hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s);
+hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 }  // namespace gm
 
 using namespace gm;
@@ -338,6 +339,7 @@ enum TaskClass : signed char { kClsOrthoClassed = 0, kClsOrthoOther = 1, kClsGen
 enum MergedList {
   kMSetup, kMOrthoA, kMOrthoB, kMGenA, kMGenB, kMOrthoRest, kMGenRest,
   kMDirectA, kMDirectB,  // classed orthorhombic tasks of levels without collocation tiles
+  kMGenDirA, kMGenDirB,  // general tasks of levels without collocation tiles
   kMNum
 };
 struct MergedPlan {
@@ -382,6 +384,9 @@ struct TaskListImpl {
   DevBuf d_gmat, d_gmat_off;                       // general cells: cxyz -> cijk matrices
   std::vector<TileInfo> tiles;
   std::map<int, TilePlan> tile_plans;              // by lp shift
+  std::vector<TileInfo> gtiles;                    // tiles of the general tasks
+  std::map<int, TilePlan> gtile_plans;             // by lp shift
+  DevBuf d_gitems;                                 // their items
   // Per-call level tables.  A call returns once its work is enqueued, so the
   // table of a call still in flight must not be rewritten: each call takes
   // the next slot of a ring (pinned staging + device copy), and waits for
@@ -431,6 +436,8 @@ struct TaskListImpl {
                       &d_gridf, &d_gmat, &d_gmat_off};
     for (auto &kv : tile_plans) kv.second.d.release();
     for (auto &kv : merged_plans) kv.second.d.release();
+    for (auto &kv : gtile_plans) kv.second.d.release();
+    d_gitems.release();
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
     for (LevelSlot &ls : level_slots) {
@@ -915,6 +922,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->tile_plans.clear();
   for (auto &kv : tl->merged_plans) kv.second.d.release();
   tl->merged_plans.clear();
+  for (auto &kv : tl->gtile_plans) kv.second.d.release();
+  tl->gtile_plans.clear();
   for (int l = 0; l < nlevels; l++) {
     OrthoLevel &o = tl->olev[l];
     o.tiled = false;
@@ -1006,6 +1015,100 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     if (g.end == 0) g.begin = s;
     g.end = s + 1;
   }
+  // --- collocation tiles of the general tasks: full rows of stride nx | 1;
+  //     items are runs of rows j and planes k of a task's index box whose
+  //     grid index stays inside the border-masked grid and one tile ---
+  std::vector<TileItem> gitems;
+  tl->gtiles.clear();
+  for (int l = 0; l < nlevels; l++) {
+    OrthoLevel &g = tl->glev[l];
+    g.tiled = false;
+    if (g.end <= g.begin) continue;
+    const Layout &y = tl->layouts[l];
+    const long long nx = y.npts_local[0], ny = y.npts_local[1], nz = y.npts_local[2];
+    const long long sx = nx | 1;
+    const long long wave = 8LL * (84 + 343) + 16;  // lp <= 6: packed cxyz + dense cijk
+    const long long cap = std::min(kTileDoubles, (lds_limit() - 8 * wave) / 8);
+    if (sx * ny * nz <= cap) {
+      g.by = (int)ny;
+      g.bz = (int)nz;
+    } else if (cap / (sx * nz) >= 4) {
+      g.by = (int)(cap / (sx * nz));
+      g.bz = (int)nz;
+    } else if (sx * 8 * 2 <= cap) {
+      g.by = (int)std::min<long long>(8, ny);
+      g.bz = (int)std::min<long long>(nz, cap / (sx * g.by));
+    } else {
+      continue;
+    }
+    bool fits = true;
+    for (int s2 = g.begin; s2 < g.end && fits; s2++)
+      for (int d = 1; d < 3; d++)
+        fits = fits && tasks[s2].lo[d] > -32768 && tasks[s2].hi[d] < 32767;
+    if (!fits) continue;
+    g.tiled = true;
+    const int nty = (int)((ny + g.by - 1) / g.by);
+    struct Seg { int a, b, t; };
+    auto segments = [&](const Task &T, int d, int bsz, std::vector<Seg> &out) {
+      out.clear();
+      int prev = -2;
+      bool open = false;
+      for (int c = T.lo[d]; c <= T.hi[d]; c++) {
+        const int gg = pmod(c - y.shift_local[d], y.npts_global[d]);
+        if (gg < T.blo[d] || gg > T.bhi[d]) {
+          open = false;
+          continue;
+        }
+        const int t = gg / bsz;
+        if (!open || t != out.back().t || gg != prev + 1)
+          out.push_back(Seg{c, c, t});
+        else
+          out.back().b = c;
+        open = true;
+        prev = gg;
+      }
+    };
+    struct Keyed { long long key; TileItem it; };
+    std::vector<Keyed> lev_items;
+    std::vector<Seg> ys, zs;
+    for (int s2 = g.begin; s2 < g.end; s2++) {
+      const Task &T = tasks[s2];
+      segments(T, 1, g.by, ys);
+      segments(T, 2, g.bz, zs);
+      for (const Seg &a : ys)
+        for (const Seg &b : zs) {
+          TileItem it{};
+          it.task = s2;
+          it.j0 = (short)a.a;
+          it.j1 = (short)a.b;
+          it.k0 = (short)b.a;
+          it.k1 = (short)b.b;
+          lev_items.push_back(Keyed{((long long)b.t * nty + a.t) << 8 | tl->lp0s[s2], it});
+        }
+    }
+    std::stable_sort(lev_items.begin(), lev_items.end(),
+                     [](const Keyed &a, const Keyed &b) { return a.key < b.key; });
+    for (size_t a = 0; a < lev_items.size();) {
+      const long long tile = lev_items[a].key >> 8;
+      size_t b = a;
+      while (b < lev_items.size() && (lev_items[b].key >> 8) == tile) b++;
+      TileInfo ti{};
+      ti.level = l;
+      ti.gy0 = (int)(tile % nty) * g.by;
+      ti.gz0 = (int)(tile / nty) * g.bz;
+      const int first = (int)gitems.size();
+      for (size_t q = a; q < b; q++) gitems.push_back(lev_items[q].it);
+      for (int v = 0; v <= kLp0Slots; v++) {
+        int q = first;
+        while (q < (int)gitems.size() && tl->lp0s[gitems[q].task] < v) q++;
+        ti.start[v] = q;
+      }
+      tl->gtiles.push_back(ti);
+      a = b;
+    }
+  }
+  upload(tl->d_gitems, gitems);
+
   for (auto *lv : {&tl->olev, &tl->glev})
     for (auto &o : *lv)
       for (int v = 0; v < kLp0Slots; v++) {
@@ -1215,10 +1318,13 @@ int cxyz_stride(const TaskListImpl *tl, int sh) {
 // an lp bucket are dealt to a number of workgroups proportional to their
 // count, about four workgroups per CU over a level (one is resident per CU;
 // several rounds keep the last one short).
-TilePlan &tile_plan(TaskListImpl *tl, int sh) {
-  auto it = tl->tile_plans.find(sh);
-  if (it != tl->tile_plans.end()) return it->second;
-  TilePlan &plan = tl->tile_plans[sh];
+TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false) {
+  auto &plans = general ? tl->gtile_plans : tl->tile_plans;
+  const std::vector<TileInfo> &tiles = general ? tl->gtiles : tl->tiles;
+  const std::vector<OrthoLevel> &lev = general ? tl->glev : tl->olev;
+  auto it = plans.find(sh);
+  if (it != plans.end()) return it->second;
+  TilePlan &plan = plans[sh];
   plan.range.assign(tl->nlevels, {std::make_pair(0, 0), std::make_pair(0, 0)});
   std::vector<TileUnit> units;
   auto slot = [](int v) { return std::min(std::max(v, 0), kLp0Slots); };
@@ -1226,26 +1332,25 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh) {
     for (int r = 0; r < 2; r++) {
       const int v0 = slot(r == 0 ? 0 : 5 - sh), v1 = slot(r == 0 ? 5 - sh : 7 - sh);
       long long total = 0;
-      for (const TileInfo &t : tl->tiles)
+      for (const TileInfo &t : tiles)
         if (t.level == l) total += t.start[v1] - t.start[v0];
       const int first = (int)units.size();
       if (total > 0) {
         const long long per =
             std::max<long long>(32, (total + 4LL * num_cus() - 1) / (4LL * num_cus()));
-        for (const TileInfo &t : tl->tiles) {
+        for (const TileInfo &t : tiles) {
           if (t.level != l) continue;
           const int a = t.start[v0], b = t.start[v1];
           if (b <= a) continue;
           const int nw = (int)std::max<long long>(1, ((long long)(b - a) + per / 2) / per);
           for (int w = 0; w < nw; w++)
-            units.push_back(TileUnit{a, b, w, nw, t.gy0, t.gz0, l, tl->olev[l].by, tl->olev[l].bz});
+            units.push_back(TileUnit{a, b, w, nw, t.gy0, t.gz0, l, lev[l].by, lev[l].bz});
         }
       }
       plan.range[l][r] = std::make_pair(first, (int)units.size() - first);
     }
   }
   // merged launches: the units of every level, bucket by bucket
-  const int nbase = (int)units.size();
   for (int r = 0; r < 2; r++) {
     const int first = (int)units.size();
     for (int l = 0; l < tl->nlevels; l++)
@@ -1253,7 +1358,6 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh) {
         units.push_back(units[plan.range[l][r].first + u]);
     plan.mrange[r] = std::make_pair(first, (int)units.size() - first);
   }
-  (void)nbase;
   upload(plan.d, units);
   return plan;
 }
@@ -1343,6 +1447,55 @@ WavePlan general_plan(int lp) {
   return w;
 }
 
+// LDS plan of a general tile launch over `level` (-1: every tiled general
+// level): the largest tile (row stride nx | 1) and wave regions holding the
+// packed cxyz and the dense cijk.
+bool general_tile_plan(const TaskListImpl *tl, int level, int lp, WavePlan &w) {
+  w = WavePlan{};
+  w.level = level;
+  w.np = lp + 1;
+  w.o_cxyz = 0;
+  w.o_u = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  w.wave_bytes = (8 * (w.o_u + w.np * w.np * w.np) + 15) & ~15;
+  long long win = 0;
+  for (int l = 0; l < tl->nlevels; l++)
+    if ((level < 0 || l == level) && tl->glev[l].tiled)
+      win = std::max(win, (long long)(tl->npts_local[3 * l] | 1) * tl->glev[l].by * tl->glev[l].bz);
+  if (win <= 0) return false;
+  const int most = lp <= 4 ? 12 : 8;
+  for (int nw : {12, 10, 8, 6, 4})
+    if (nw <= most && 8 * win + (long long)nw * w.wave_bytes <= lds_limit()) {
+      w.window = (int)win;
+      w.nwaves = nw;
+      w.bytes = 8 * w.window + nw * w.wave_bytes;
+      return true;
+    }
+  return false;
+}
+
+// General tiles of one level (or all levels: level -1, unit range of the
+// merged arrangement), bucket r of lp shift sh.
+void launch_general_tile_bucket(TaskListImpl *tl, const PointParams &P, int level, int r, int lp,
+                                hipStream_t s) {
+  const int sh = P.ld[0] + P.ld[2];
+  const TilePlan &tp = tile_plan(tl, sh, true);
+  const std::pair<int, int> rg = level < 0 ? tp.mrange[r] : tp.range[level][r];
+  if (rg.second <= 0) return;
+  const int lpb = r == 0 ? 4 : 6;
+  PointParams R = P;
+  R.units = (const TileUnit *)tp.d.p;
+  R.items = (const TileItem *)tl->d_gitems.p;
+  R.unit_base = rg.first;
+  if (!general_tile_plan(tl, level, std::min(lpb, lp), R.wp)) {
+    if (level >= 0) fatal("level %d: no room for the general collocation tile in LDS", level + 1);
+    for (int l = 0; l < tl->nlevels; l++)  // the largest tile and tables do not fit together
+      if (tl->glev[l].tiled) launch_general_tile_bucket(tl, P, l, r, lp, s);
+    return;
+  }
+  TimedLaunch tm(13, s);
+  HIP_CHECK(launch_general_tiles(R, lpb, rg.second, s));
+}
+
 // General tasks, level by level: wave-per-task kernels for lp <= 6 (cab ->
 // cxyz pass, point kernel; integrate: point kernel, cxyz -> hab pass), the
 // per-task kernels for larger lp.
@@ -1374,6 +1527,10 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         const int a = cuts[r][0], b = cuts[r][1];
         if (b <= a) continue;
         const int lpb = r == 0 ? 4 : 6;
+        if (mode == kCollocate && g.tiled) {
+          launch_general_tile_bucket(tl, P, l, r, tl->lp0s[b - 1] + sh, s);
+          continue;
+        }
         PointParams R = P;
         R.task_base = a;
         R.ntasks = b - a;
@@ -1413,6 +1570,7 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
       if (wave && lp <= 6) {
         lists[kMSetup].push_back(t);
         lists[lp <= 4 ? kMGenA : kMGenB].push_back(t);
+        if (!tl->glev[tl->task_level[t]].tiled) lists[lp <= 4 ? kMGenDirA : kMGenDirB].push_back(t);
       } else {
         lists[kMGenRest].push_back(t);
       }
@@ -1513,10 +1671,13 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         HIP_CHECK(launch_collocate_wave(R, lpb, s));
       }
     }
-    for (int m : {kMGenA, kMGenB}) {
+    if (mode == kCollocate)
+      for (int r = 0; r < 2; r++) launch_general_tile_bucket(tl, P, -1, r, mp.lp[r == 0 ? kMGenA : kMGenB], s);
+    for (int m : (mode == kIntegrate ? std::array<int, 2>{kMGenA, kMGenB}
+                                     : std::array<int, 2>{kMGenDirA, kMGenDirB})) {
       PointParams R = sub(m);
       if (R.ntasks <= 0) continue;
-      const int lpb = m == kMGenA ? 4 : 6;
+      const int lpb = (m == kMGenA || m == kMGenDirA) ? 4 : 6;
       R.wp = general_plan(std::min(lpb, mp.lp[m]));
       TimedLaunch tm(13, s);
       HIP_CHECK(mode == kIntegrate ? launch_general_integrate(R, lpb, s)

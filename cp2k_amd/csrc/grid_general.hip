@@ -35,8 +35,9 @@ struct GenLds {
   double *cijk;  // dense [np][np][np], then (collocate) the staging of GenStage
 };
 
+// Wave regions follow the LDS tile window (wp.window doubles, 0 without tiles).
 __device__ __forceinline__ GenLds gen_lds(double *lds, const WavePlan &wp, int wave) {
-  double *b = reinterpret_cast<double *>(reinterpret_cast<char *>(lds) +
+  double *b = reinterpret_cast<double *>(reinterpret_cast<char *>(lds) + 8 * (size_t)wp.window +
                                          (size_t)wave * wp.wave_bytes);
   return GenLds{b + wp.o_cxyz, b + wp.o_u};
 }
@@ -193,6 +194,69 @@ __device__ __forceinline__ void gen_collocate_rows(const Task &T, const Layout &
   }
 }
 
+// Row stride (doubles) of a general LDS tile: odd, so that lanes on
+// consecutive rows hit different LDS banks.
+__device__ __forceinline__ int gen_tile_stride(int nx) { return nx | 1; }
+
+// Collocate the rows j0..j1, planes k0..k1 of a task (an item) into the LDS
+// tile of unit U: win[((gz - gz0) * by + gy - gy0) * sx + gx].
+template <int LP>
+__device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, const double *cijk,
+                                              int NP, const TileItem &I, const TileUnit &U,
+                                              double *win, int lane) {
+  const int nj = I.j1 - I.j0 + 1, nk = I.k1 - I.k0 + 1;
+  const int ngx = lay.npts_global[0];
+  const int sx = gen_tile_stride(lay.npts_local[0]);
+  double hh = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
+  const double r2step = exp(-2.0 * T.zetp * hh);
+  for (int r = lane; r < nj * nk; r += 64) {
+    const int j = I.j0 + r % nj, k = I.k0 + r / nj;
+    int i0, i1;
+    long long base;
+    if (!gen_row(T, lay, j, k, i0, i1, base)) continue;
+    const int ly = wrap(j - lay.shift_local[1], lay.npts_global[1]) - U.gy0;
+    const int lz = wrap(k - lay.shift_local[2], lay.npts_global[2]) - U.gz0;
+    if ((unsigned)ly >= (unsigned)U.by || (unsigned)lz >= (unsigned)U.bz) continue;  // not for host-built items
+    double *row = win + (lz * U.by + ly) * sx;
+    const double dj = j - T.gp[1], dk = k - T.gp[2];
+    double ci[LP + 1];
+#pragma unroll
+    for (int il = 0; il <= LP; il++) ci[il] = 0.0;
+    double dkp = 1.0;
+#pragma unroll
+    for (int kl = 0; kl <= LP; kl++) {
+      double w = dkp;
+#pragma unroll
+      for (int jl = 0; jl <= LP - kl; jl++) {
+        const double *c = cijk + (kl * NP + jl) * NP;
+#pragma unroll
+        for (int il = 0; il <= LP - kl - jl; il++) ci[il] = fma(c[il], w, ci[il]);
+        w *= dj;
+      }
+      dkp *= dk;
+    }
+    int gx = wrap(i0 - lay.shift_local[0], ngx);
+    for (int s0 = i0; s0 <= i1; s0 += kRestart) {
+      double g, q;
+      gen_gauss(T, lay, s0, j, k, g, q);
+      const int s1 = min(i1, s0 + kRestart - 1);
+      double di = s0 - T.gp[0];
+      for (int i = s0; i <= s1; i++) {
+        double p = ci[LP];
+#pragma unroll
+        for (int il = LP - 1; il >= 0; il--) p = fma(p, di, ci[il]);
+        if (gx >= T.blo[0] && gx <= T.bhi[0]) atomicAdd(&row[gx], p * g);
+        g *= q;
+        q *= r2step;
+        di += 1.0;
+        gx = (gx + 1 == ngx) ? 0 : gx + 1;
+      }
+    }
+  }
+}
+
 // Integrate: lane-private acc[(kl, jl, il)] (packed, il fastest) of the
 // task's rows; R is the type the potential is read in.
 template <int LP, typename R>
@@ -308,6 +372,55 @@ __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__re
   }
 }
 
+// Collocation of general tasks through LDS tiles: workgroup = unit
+// P.unit_base + blockIdx.x, the items U.ibeg + U.first + m * U.stride < U.iend
+// of one tile of level U.level; the tile is added to HBM once.
+template <int LPB>
+__global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
+    const Task *__restrict__ tasks, const Level *__restrict__ levels,
+    const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  const TileUnit U = units[P.unit_base + blockIdx.x];
+  const Level &L = levels[U.level];
+  const Layout &lay = L.lay;
+  const int nx = lay.npts_local[0], ny = lay.npts_local[1], nz = lay.npts_local[2];
+  const int sx = gen_tile_stride(nx);
+  const int window = sx * U.by * U.bz;  // <= P.wp.window
+  for (int e = threadIdx.x; e < window; e += blockDim.x) lds[e] = 0.0;
+  __syncthreads();
+  const int NP = P.wp.np;
+  const GenLds W = gen_lds(lds, P.wp, wave);
+  for (long long it = U.ibeg + U.first + (long long)wave * U.stride; it < U.iend;
+       it += (long long)nw * U.stride) {
+    const TileItem I = items[it];
+    const Task &T = tasks[I.task];
+    const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+    gen_cijk(P, T, I.task, lp, W, NP, lane);
+    switch (lp) {
+      case 0: gen_tile_rows<0>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 1: gen_tile_rows<1>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 2: gen_tile_rows<2>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 3: gen_tile_rows<3>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 4: gen_tile_rows<4>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 5: if (LPB >= 5) gen_tile_rows<(LPB >= 5 ? 5 : 0)>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 6: if (LPB >= 6) gen_tile_rows<(LPB >= 6 ? 6 : 0)>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      default: break;  // the host routes larger lp to the per-task kernels
+    }
+    wave_sync();
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < window; e += blockDim.x) {
+    const double v = lds[e];
+    if (v == 0.0) continue;
+    const int x = e % sx, r = e / sx;
+    const int y = U.gy0 + r % U.by, z = U.gz0 + r / U.by;
+    if (x < nx && y < ny && z < nz) atomicAdd(&L.grid[((long long)z * ny + y) * nx + x], v);
+  }
+}
+
 // Integration of general tasks, one task per wave; writes the task's cxyz
 // (packed) for the cxyz -> hab pass.  R = float reads Level::gridf.
 template <int LPB, typename R>
@@ -363,6 +476,28 @@ hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.np > lpb + 1) return hipErrorInvalidValue;
   if (lpb == 4) return launch_gen(general_collocate_kernel<4>, P, s);
   if (lpb == 6) return launch_gen(general_collocate_kernel<6>, P, s);
+  return hipErrorInvalidValue;
+}
+
+// nunits workgroups from unit P.unit_base; P.wp.window: the largest tile.
+hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s) {
+  if (nunits <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0 ||
+      P.wp.np > lpb + 1)
+    return hipErrorInvalidValue;
+  auto go = [&](auto kernel) {
+    const size_t lds = (size_t)P.wp.bytes;
+    if (lds > 65536) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kernel, dim3(nunits), dim3(64 * P.wp.nwaves), lds, s, P.tasks, P.levels,
+                       P.units, P.items, P);
+    return hipGetLastError();
+  };
+  if (lpb == 4) return go(general_tile_kernel<4>);
+  if (lpb == 6) return go(general_tile_kernel<6>);
   return hipErrorInvalidValue;
 }
 
