@@ -372,15 +372,19 @@ def random_density(args, rng, spd_dim_limit=4000):
         get = lambda i, j: P[first[i - 1]:first[i], first[j - 1]:first[j]]  # noqa: E731
     else:
         get = None
+    if get is None:
+        # independent N(0,1) entries; diagonal blocks symmetrised
+        buf[:] = rng.normal(size=args.buffer_size)
+        for b in np.nonzero(bi == bj)[0]:
+            n = nsgf[kinds[int(bi[b]) - 1] - 1]
+            off = int(args.block_offsets[b])
+            blk = buf[off:off + n * n].reshape(n, n)
+            buf[off:off + n * n] = (0.5 * (blk + blk.T)).ravel()
+        return buf
     for b in range(len(args.block_offsets)):
         i, j = int(bi[b]), int(bj[b])
         ni, nj = nsgf[kinds[i - 1] - 1], nsgf[kinds[j - 1] - 1]
-        if get is not None:
-            blk = get(i, j)
-        else:
-            blk = rng.normal(size=(ni, nj))
-            if i == j:
-                blk = 0.5 * (blk + blk.T)
+        blk = get(i, j)
         off = int(args.block_offsets[b])
         buf[off:off + ni * nj] = blk.T.ravel()  # column-major, rows = atom i
     return buf
