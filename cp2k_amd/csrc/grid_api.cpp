@@ -328,8 +328,8 @@ struct TileInfo {
 // (lp <= 4, lp 5..6) a range of TileUnit records in `d`.
 struct TilePlan {
   DevBuf d;
-  std::vector<std::array<std::pair<int, int>, 2>> range;  // [level][bucket] = (first, count)
-  std::pair<int, int> mrange[2];                          // [bucket]: all levels (merged)
+  std::vector<std::array<std::pair<int, int>, 3>> range;  // [level][bucket] = (first, count)
+  std::pair<int, int> mrange[3];                          // [bucket]: all levels (merged)
 };
 
 // Launch class of a task in merged mode.
@@ -340,6 +340,8 @@ enum MergedList {
   kMSetup, kMOrthoA, kMOrthoB, kMGenA, kMGenB, kMOrthoRest, kMGenRest,
   kMDirectA, kMDirectB,  // classed orthorhombic tasks of levels without collocation tiles
   kMGenDirA, kMGenDirB,  // general tasks of levels without collocation tiles
+  kMSetupHi, kMOrthoC, kMDirectC,  // collocation of classed orthorhombic tasks with lp 7..8
+  kMOrthoRestC,          // orthorhombic tasks the per-task kernels collocate
   kMNum
 };
 struct MergedPlan {
@@ -931,8 +933,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     const Layout &y = tl->layouts[l];
     const long long nx = y.npts_local[0], ny = y.npts_local[1], nz = y.npts_local[2];
     // the tile shares LDS with at least four wave regions of the largest
-    // tables (lp <= 6, this level's cube extent)
-    const long long wave_max = 8LL * (84 + 3LL * o.max_ext * 7) + 16;
+    // tables (lp <= 8, this level's cube extent)
+    const long long wave_max = 8LL * (165 + 3LL * o.max_ext * 9) + 16;
     const long long cap = std::min(kTileDoubles, (lds_limit() - 4 * wave_max) / 8);
     if (nx * ny * nz <= cap) {
       o.by = (int)ny;
@@ -1310,7 +1312,7 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
 
 // Doubles per task of the packed cxyz buffer for a call with lp shift sh.
 int cxyz_stride(const TaskListImpl *tl, int sh) {
-  const int lp = std::min(6, tl->max_lp + sh);
+  const int lp = std::min(8, tl->max_lp + sh);
   return (lp + 1) * (lp + 2) * (lp + 3) / 6;
 }
 
@@ -1325,12 +1327,14 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false) {
   auto it = plans.find(sh);
   if (it != plans.end()) return it->second;
   TilePlan &plan = plans[sh];
-  plan.range.assign(tl->nlevels, {std::make_pair(0, 0), std::make_pair(0, 0)});
+  plan.range.assign(tl->nlevels, {std::make_pair(0, 0), std::make_pair(0, 0), std::make_pair(0, 0)});
   std::vector<TileUnit> units;
   auto slot = [](int v) { return std::min(std::max(v, 0), kLp0Slots); };
   for (int l = 0; l < tl->nlevels; l++) {
-    for (int r = 0; r < 2; r++) {
-      const int v0 = slot(r == 0 ? 0 : 5 - sh), v1 = slot(r == 0 ? 5 - sh : 7 - sh);
+    for (int r = 0; r < 3; r++) {
+      // lp <= 4, lp 5..6, lp 7..8
+      const int v0 = slot(r == 0 ? 0 : (r == 1 ? 5 - sh : 7 - sh));
+      const int v1 = slot(r == 0 ? 5 - sh : (r == 1 ? 7 - sh : 9 - sh));
       long long total = 0;
       for (const TileInfo &t : tiles)
         if (t.level == l) total += t.start[v1] - t.start[v0];
@@ -1351,7 +1355,7 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false) {
     }
   }
   // merged launches: the units of every level, bucket by bucket
-  for (int r = 0; r < 2; r++) {
+  for (int r = 0; r < 3; r++) {
     const int first = (int)units.size();
     for (int l = 0; l < tl->nlevels; l++)
       for (int u = 0; u < plan.range[l][r].second; u++)
@@ -1374,13 +1378,15 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
     const OrthoLevel &o = tl->olev[l];
     if (o.end <= o.begin) continue;
     auto first = [&](int v) { return o.lp_start[std::min(std::max(v, 0), kLp0Slots - 1)]; };
-    int c1 = first(5 - sh), c2 = first(7 - sh);  // lp <= 4 | lp 5..6 | larger
-    if (!wave_kernels() || !o.classed) c1 = c2 = o.begin;
-    if (c2 > o.begin) {
-      const int lp_all = std::min(6, tl->lp0s[c2 - 1] + sh);
+    // lp <= 4 | lp 5..6 | lp 7..8 (collocate only) | larger
+    int c1 = first(5 - sh), c2 = first(7 - sh), c3 = first(9 - sh);
+    if (!wave_kernels() || !o.classed) c1 = c2 = c3 = o.begin;
+    if (mode != kCollocate) c3 = c2;
+    if (c3 > o.begin) {
+      const int lp_all = std::min(8, tl->lp0s[c3 - 1] + sh);
       PointParams Q = P;
       Q.task_base = o.begin;
-      Q.ntasks = c2 - o.begin;
+      Q.ntasks = c3 - o.begin;
       Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanSetup);
       if (mode == kCollocate) {
         TimedLaunch tm(6, s);
@@ -1389,14 +1395,14 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
                         P.ld[3] == 0;
         Q.lane_cxyz = ab ? 1 : 0;
         if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
-        if (!ab || tl->highl_before[c2] > tl->highl_before[o.begin])
+        if (!ab || tl->highl_before[c3] > tl->highl_before[o.begin])
           HIP_CHECK(launch_cxyz_wave(Q, s));
       }
-      const int cuts[2][2] = {{o.begin, c1}, {c1, c2}};
-      for (int r = 0; r < 2; r++) {
+      const int cuts[3][2] = {{o.begin, c1}, {c1, c2}, {c2, c3}};
+      for (int r = 0; r < 3; r++) {
         const int a = cuts[r][0], b = cuts[r][1];
         if (b <= a) continue;
-        const int lpb = r == 0 ? 4 : 6;
+        const int lpb = r == 0 ? 4 : (r == 1 ? 6 : 8);
         const int lp = std::min(lpb, tl->lp0s[b - 1] + sh);
         PointParams R = P;
         R.task_base = a;
@@ -1424,10 +1430,10 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         HIP_CHECK(launch_finish_wave(Q, s));
       }
     }
-    if (o.end > c2) {
+    if (o.end > c3) {
       PointParams Q = P;
-      Q.task_base = c2;
-      Q.ntasks = o.end - c2;
+      Q.task_base = c3;
+      Q.ntasks = o.end - c3;
       TimedLaunch tm(11, s);
       HIP_CHECK(mode == kIntegrate ? launch_integrate(Q, true, s) : launch_collocate(Q, true, s));
     }
@@ -1578,8 +1584,14 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
       lists[kMSetup].push_back(t);
       lists[lp <= 4 ? kMOrthoA : kMOrthoB].push_back(t);
       if (!tl->olev[tl->task_level[t]].tiled) lists[lp <= 4 ? kMDirectA : kMDirectB].push_back(t);
+    } else if (c == kClsOrthoClassed && wave && lp <= 8) {
+      lists[kMSetupHi].push_back(t);
+      lists[kMOrthoC].push_back(t);
+      if (!tl->olev[tl->task_level[t]].tiled) lists[kMDirectC].push_back(t);
+      lists[kMOrthoRest].push_back(t);  // integrate: per-task kernels
     } else {
       lists[kMOrthoRest].push_back(t);
+      lists[kMOrthoRestC].push_back(t);
     }
   }
   std::vector<int> all;
@@ -1609,7 +1621,7 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
     return Q;
   };
   PointParams Q = sub(kMSetup);
-  if (Q.ntasks > 0) {
+  {
     Q.wp = make_wave_plan(tl, -1, std::min(6, mp.lp[kMSetup]), P.ld, kPlanSetup);
     if (mode == kCollocate) {
       TimedLaunch tm(6, s);
@@ -1618,6 +1630,12 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       Q.lane_cxyz = ab ? 1 : 0;
       if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
       HIP_CHECK(launch_cxyz_wave(Q, s));
+      PointParams H = sub(kMSetupHi);  // lp 7..8
+      if (H.ntasks > 0) {
+        H.wp = make_wave_plan(tl, -1, std::min(8, mp.lp[kMSetupHi]), P.ld, kPlanSetup);
+        H.lane_cxyz = 0;
+        HIP_CHECK(launch_cxyz_wave(H, s));
+      }
     }
     if (mode == kIntegrate) {
       for (int m : {kMOrthoA, kMOrthoB}) {
@@ -1637,10 +1655,10 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         if (tl->olev[l].tiled)
           max_window = std::max(max_window, (long long)tl->npts_local[3 * l] * tl->olev[l].by *
                                                 tl->olev[l].bz);
-      for (int r = 0; r < 2; r++) {
+      for (int r = 0; r < 3; r++) {
         if (tp.mrange[r].second <= 0) continue;
-        const int lpb = r == 0 ? 4 : 6;
-        const int lp = std::min(lpb, mp.lp[r == 0 ? kMOrthoA : kMOrthoB]);
+        const int lpb = r == 0 ? 4 : (r == 1 ? 6 : 8);
+        const int lp = std::min(lpb, mp.lp[r == 0 ? kMOrthoA : (r == 1 ? kMOrthoB : kMOrthoC)]);
         PointParams R = P;
         R.units = (const TileUnit *)tp.d.p;
         const int np = lp + 1;
@@ -1662,10 +1680,10 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
           }
         }
       }
-      for (int m : {kMDirectA, kMDirectB}) {
+      for (int m : {kMDirectA, kMDirectB, kMDirectC}) {
         PointParams R = sub(m);
         if (R.ntasks <= 0) continue;
-        const int lpb = m == kMDirectA ? 4 : 6;
+        const int lpb = m == kMDirectA ? 4 : (m == kMDirectB ? 6 : 8);
         R.wp = make_wave_plan(tl, -1, std::min(lpb, mp.lp[m]), P.ld, kPlanPoints);
         TimedLaunch tm(8, s);
         HIP_CHECK(launch_collocate_wave(R, lpb, s));
@@ -1689,11 +1707,11 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       HIP_CHECK(launch_finish_wave(Q, s));
     }
   }
-  for (int m : {kMOrthoRest, kMGenRest}) {
+  for (int m : {mode == kIntegrate ? kMOrthoRest : kMOrthoRestC, kMGenRest}) {
     PointParams R = sub(m);
     if (R.ntasks <= 0) continue;
     TimedLaunch tm(11, s);
-    const bool ortho = m == kMOrthoRest;
+    const bool ortho = m != kMGenRest;
     HIP_CHECK(mode == kIntegrate ? launch_integrate(R, ortho, s) : launch_collocate(R, ortho, s));
   }
 }

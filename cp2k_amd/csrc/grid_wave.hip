@@ -162,6 +162,28 @@ __device__ __forceinline__ void axis_offsets(const Task &T, const Layout &lay, i
   cw[2] = pm(T.center[2] - lay.shift_local[2], lay.npts_global[2]);
 }
 
+// High lp (7, 8): the same sums with the ly loop kept rolled and py read
+// from the column's LDS table row ry, which keeps the register count below
+// the limit of the tile kernels.
+template <int LP, typename R>
+__device__ __forceinline__ void column_poly_ry(const R *cx, const R *px, const R *ry, R *D) {
+  int e = 0;
+#pragma unroll
+  for (int lz = 0; lz <= LP; lz++) {
+    R s = 0;
+#pragma unroll 1
+    for (int ly = 0; ly <= LP - lz; ly++) {
+      R t = 0;
+#pragma unroll
+      for (int lx = 0; lx <= LP - lz; lx++)
+        if (lx <= LP - lz - ly) t = fmar(cx[e + lx], px[lx], t);
+      e += LP - lz - ly + 1;
+      s = fmar(t, ry[ly], s);
+    }
+    D[lz] = s;
+  }
+}
+
 // D[lz] = sum_ly py[ly] sum_lx cxyz[(lz, ly, lx)] px[lx] of one column.
 template <int LP, typename R>
 __device__ __forceinline__ void column_poly(const R *cx, const R *px, const R *py, R *D) {
@@ -193,9 +215,19 @@ __device__ __forceinline__ void collocate_columns(const PointParams &P, const Ta
   axis_offsets(T, lay, cw);
   for (int c0 = 0; c0 < G.ncol; c0 += 64) {
     const Column C = load_column(P, G, lay, cw, c0 + lane);
-    R px[LP + 1], py[LP + 1], D[LP + 1];
-    load_pxy<LP>(T, wp, tab, C, px, py);
-    column_poly<LP>(cx, px, py, D);
+    R D[LP + 1];
+    if constexpr (LP >= 7) {
+      R px[LP + 1];
+      const R *rx = tab + clampi(C.i - T.lo[0], 0, wp.ext - 1) * NP;
+      const R *ry = tab + (wp.ext + clampi(C.j - T.lo[1], 0, wp.ext - 1)) * NP;
+#pragma unroll
+      for (int l = 0; l <= LP; l++) px[l] = rx[l];
+      column_poly_ry<LP>(cx, px, ry, D);
+    } else {
+      R px[LP + 1], py[LP + 1];
+      load_pxy<LP>(T, wp, tab, C, px, py);
+      column_poly<LP>(cx, px, py, D);
+    }
     const int kmin = P.geo_kmin[G.grp_off + (c0 >> 6)];
     const unsigned span = (unsigned)(2 - 2 * C.kl);  // 0 for an idle lane
     int gz = wrap(cw[2] + kmin, ngz);
@@ -242,9 +274,18 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
       else
         C[u].kl = 1;  // outside the tile: never for items built on the host
     }
-    R px[LP + 1], py[LP + 1];
-    load_pxy<LP>(T, wp, tab, C[u], px, py);
-    column_poly<LP>(cx, px, py, D[u]);
+    if constexpr (LP >= 7) {
+      R px[LP + 1];
+      const R *rx = tab + clampi(C[u].i - T.lo[0], 0, wp.ext - 1) * wp.np;
+      const R *ry = tab + (wp.ext + clampi(C[u].j - T.lo[1], 0, wp.ext - 1)) * wp.np;
+#pragma unroll
+      for (int l = 0; l <= LP; l++) px[l] = rx[l];
+      column_poly_ry<LP>(cx, px, ry, D[u]);
+    } else {
+      R px[LP + 1], py[LP + 1];
+      load_pxy<LP>(T, wp, tab, C[u], px, py);
+      column_poly<LP>(cx, px, py, D[u]);
+    }
     span[u] = (unsigned)(2 - 2 * C[u].kl);  // 0 for an idle lane
     klmin = min(klmin, C[u].kl);
   }
@@ -659,6 +700,8 @@ __global__ __launch_bounds__(256) void collocate_wave_kernel(const Task *__restr
     case 4: collocate_columns<4>(P, T, G, lay, wp, tab, cx, dst, lane); break;
     case 5: if (LPB >= 5) collocate_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, cx, dst, lane); break;
     case 6: if (LPB >= 6) collocate_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 7: if (LPB >= 7) collocate_columns<(LPB >= 7 ? 7 : 0)>(P, T, G, lay, wp, tab, cx, dst, lane); break;
+    case 8: if (LPB >= 8) collocate_columns<(LPB >= 8 ? 8 : 0)>(P, T, G, lay, wp, tab, cx, dst, lane); break;
     default: break;  // the host routes larger lp to the per-task kernels
   }
 }
@@ -699,6 +742,8 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
       case 4: tile_columns<4>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
       case 5: if (LPB >= 5) tile_columns<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
       case 6: if (LPB >= 6) tile_columns<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 7: if (LPB >= 7) tile_columns<(LPB >= 7 ? 7 : 0)>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
+      case 8: if (LPB >= 8) tile_columns<(LPB >= 8 ? 8 : 0)>(P, T, G, lay, wp, tab, cx, I, U, lds, lane); break;
       default: break;  // the host routes larger lp to the per-task kernels
     }
     wave_sync();
@@ -809,6 +854,7 @@ template <typename R>
 static hipError_t collocate_wave_r(const PointParams &P, int lpb, hipStream_t s) {
   if (lpb == 4) return launch_wave(collocate_wave_kernel<4, R>, P, wave_blocks(P), s, P.tasks, P.levels);
   if (lpb == 6) return launch_wave(collocate_wave_kernel<6, R>, P, wave_blocks(P), s, P.tasks, P.levels);
+  if (lpb == 8) return launch_wave(collocate_wave_kernel<8, R>, P, wave_blocks(P), s, P.tasks, P.levels);
   return hipErrorInvalidValue;
 }
 
@@ -824,6 +870,8 @@ static hipError_t collocate_tiles_r(const PointParams &P, int lpb, int nunits, h
     return launch_wave(collocate_tile_kernel<4, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
   if (lpb == 6)
     return launch_wave(collocate_tile_kernel<6, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
+  if (lpb == 8)
+    return launch_wave(collocate_tile_kernel<8, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
   return hipErrorInvalidValue;
 }
 
