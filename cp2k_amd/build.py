@@ -19,24 +19,38 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [CSRC / "grid_api.cpp", CSRC / "grid_kernels.hip", CSRC / "grid_wave.hip",
            CSRC / "grid_general.hip"]
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
 
 
-def build(verbose=False, extra=()):
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, *extra, "-o", str(tmp), *map(str, SOURCES)]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building " + LIB.name)
-    warn = [ln for ln in (res.stdout + res.stderr).splitlines()
+        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+    return [ln for ln in (res.stdout + res.stderr).splitlines()
             if "warning" in ln and "packed-fp32-ops" not in ln]
+
+
+def build(verbose=False, extra=()):
+    """Compiles every source to an object (in parallel), then links."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    objs = [objdir / (src.stem + ".o") for src in SOURCES]
+    cmds = [[HIPCC, *FLAGS, *extra, "-c", "-o", str(o), str(src)]
+            for src, o in zip(SOURCES, objs)]
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(workers) as ex:
+        warn = [w for ws in ex.map(lambda c: _run(c, verbose), cmds) for w in ws]
+    tmp = LIB.with_suffix(".so.tmp")
+    warn += _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
+                  *map(str, objs)], verbose)
     if verbose and warn:
         print("\n".join(warn))
     os.replace(tmp, LIB)

@@ -412,12 +412,16 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
     wave_sync();
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < window; e += blockDim.x) {
-    const double v = lds[e];
-    if (v == 0.0) continue;
-    const int x = e % sx, r = e / sx;
+  // one tile row per wave and pass: coalesced adds, no division per point
+  for (int r = wave; r < U.by * U.bz; r += nw) {
     const int y = U.gy0 + r % U.by, z = U.gz0 + r / U.by;
-    if (x < nx && y < ny && z < nz) atomicAdd(&L.grid[((long long)z * ny + y) * nx + x], v);
+    if (y >= ny || z >= nz) continue;
+    double *dst = L.grid + ((long long)z * ny + y) * nx;
+    const double *src = lds + r * sx;
+    for (int x = lane; x < nx; x += 64) {
+      const double v = src[x];
+      if (v != 0.0) atomicAdd(&dst[x], v);
+    }
   }
 }
 

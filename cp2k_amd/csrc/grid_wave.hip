@@ -82,11 +82,12 @@ struct Column {
   int i, j;  // cube indices
   int kl;    // planes kl .. 1 - kl; kl = 1 marks a lane without a column
   int base;  // y * nx + x of the column in the local grid
+  int gy;    // y of the column in the local grid
 };
 
 __device__ __forceinline__ Column load_column(const PointParams &P, const GeoClass &G,
                                               const Layout &lay, const int cw[3], int c) {
-  Column C{0, 0, 1, 0};
+  Column C{0, 0, 1, 0, 0};
   if (c >= G.ncol) return C;
   const int rec = P.geo_cols[G.col_off + c];
   C.i = G.lo[0] + (rec & 255);
@@ -97,6 +98,7 @@ __device__ __forceinline__ Column load_column(const PointParams &P, const GeoCla
   if (gx < lay.npts_local[0] && gy < lay.npts_local[1]) {
     C.kl = G.lo[2] + (rec >> 16);
     C.base = gy * lay.npts_local[0] + gx;
+    C.gy = gy;
   }
   return C;
 }
@@ -268,7 +270,7 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
     C[u] = load_column(P, G, lay, cw, c < c_end ? c : G.ncol);
     lbase[u] = 0;
     if (C[u].kl <= 0) {
-      const int ly = C[u].base / nx - U.gy0;  // base = y * nx + x
+      const int ly = C[u].gy - U.gy0;
       if ((unsigned)ly < (unsigned)U.by)
         lbase[u] = C[u].base - U.gy0 * nx;
       else
@@ -749,13 +751,18 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
     wave_sync();
   }
   __syncthreads();
+  // one tile row (nx points) per wave and pass: coalesced adds, no division
+  // per point
   const int nx = lay.npts_local[0], ny = lay.npts_local[1], nz = lay.npts_local[2];
-  for (int e = threadIdx.x; e < window; e += blockDim.x) {
-    const double v = lds[e];
-    if (v == 0.0) continue;
-    const int x = e % nx, r = e / nx;
+  for (int r = wave; r < U.by * U.bz; r += nw) {
     const int y = U.gy0 + r % U.by, z = U.gz0 + r / U.by;
-    if (y < ny && z < nz) atomicAdd(&L.grid[((long long)z * ny + y) * nx + x], v);
+    if (y >= ny || z >= nz) continue;
+    double *dst = L.grid + ((long long)z * ny + y) * nx;
+    const double *src = lds + r * nx;
+    for (int x = lane; x < nx; x += 64) {
+      const double v = src[x];
+      if (v != 0.0) atomicAdd(&dst[x], v);
+    }
   }
 }
 
