@@ -50,6 +50,7 @@ hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
+hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
@@ -342,6 +343,7 @@ enum MergedList {
   kMGenDirA, kMGenDirB,  // general tasks of levels without collocation tiles
   kMSetupHi, kMOrthoC, kMDirectC,  // collocation of classed orthorhombic tasks with lp 7..8
   kMOrthoRestC,          // orthorhombic tasks the per-task kernels collocate
+  kMOrthoHl,             // integration of classed orthorhombic tasks with lp 7..11
   kMNum
 };
 struct MergedPlan {
@@ -1247,7 +1249,7 @@ enum LaunchMode { kCollocate = 0, kIntegrate = 1, kCount = 2 };
 // with lp <= `lp`.  Setup/finish kernels hold dense cxyz and {cab, alpha} per
 // wave; point kernels hold the per-axis tables, and collocation keeps the
 // whole level grid in LDS when it fits next to at least four wave regions.
-enum PlanKind { kPlanSetup, kPlanPoints, kPlanWindow };
+enum PlanKind { kPlanSetup, kPlanPoints, kPlanWindow, kPlanMfma };
 
 // level -1: a merged launch over all levels (not for kPlanWindow).
 WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[4], int kind) {
@@ -1266,6 +1268,11 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
     w.o_cxyz = 0;
     w.o_u = w.np * w.np * w.np;
     nd = w.o_u + n_setup;
+  } else if (kind == kPlanMfma) {
+    // per-axis tables, then the staging of 64 columns (40 doubles each)
+    w.o_u = 0;
+    w.o_cxyz = 3 * w.ext * w.np;
+    nd = w.o_cxyz + 64 * 40;
   } else {
     // packed cxyz of the task (collocate) + per-axis tables
     w.o_cxyz = 0;
@@ -1312,7 +1319,7 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
 
 // Doubles per task of the packed cxyz buffer for a call with lp shift sh.
 int cxyz_stride(const TaskListImpl *tl, int sh) {
-  const int lp = std::min(8, tl->max_lp + sh);
+  const int lp = std::min(11, tl->max_lp + sh);
   return (lp + 1) * (lp + 2) * (lp + 3) / 6;
 }
 
@@ -1378,12 +1385,12 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
     const OrthoLevel &o = tl->olev[l];
     if (o.end <= o.begin) continue;
     auto first = [&](int v) { return o.lp_start[std::min(std::max(v, 0), kLp0Slots - 1)]; };
-    // lp <= 4 | lp 5..6 | lp 7..8 (collocate only) | larger
-    int c1 = first(5 - sh), c2 = first(7 - sh), c3 = first(9 - sh);
+    // lp <= 4 | lp 5..6 | lp 7..8 (collocate) or 7..11 (integrate) | larger
+    int c1 = first(5 - sh), c2 = first(7 - sh);
+    int c3 = mode == kCollocate ? first(9 - sh) : first(12 - sh);
     if (!wave_kernels() || !o.classed) c1 = c2 = c3 = o.begin;
-    if (mode != kCollocate) c3 = c2;
     if (c3 > o.begin) {
-      const int lp_all = std::min(8, tl->lp0s[c3 - 1] + sh);
+      const int lp_all = std::min(mode == kCollocate ? 8 : 11, tl->lp0s[c3 - 1] + sh);
       PointParams Q = P;
       Q.task_base = o.begin;
       Q.ntasks = c3 - o.begin;
@@ -1402,15 +1409,15 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       for (int r = 0; r < 3; r++) {
         const int a = cuts[r][0], b = cuts[r][1];
         if (b <= a) continue;
-        const int lpb = r == 0 ? 4 : (r == 1 ? 6 : 8);
+        const int lpb = r == 0 ? 4 : (r == 1 ? 6 : (mode == kIntegrate ? 11 : 8));
         const int lp = std::min(lpb, tl->lp0s[b - 1] + sh);
         PointParams R = P;
         R.task_base = a;
         R.ntasks = b - a;
         if (mode == kIntegrate) {
-          R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanPoints);
+          R.wp = make_wave_plan(tl, l, lp, P.ld, r == 2 ? kPlanMfma : kPlanPoints);
           TimedLaunch tm(9, s);
-          HIP_CHECK(launch_integrate_wave(R, lpb, s));
+          HIP_CHECK(r == 2 ? launch_integrate_mfma(R, s) : launch_integrate_wave(R, lpb, s));
         } else if (o.tiled) {
           R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanWindow);
           R.unit_base = plan->range[l][r].first;
@@ -1584,11 +1591,15 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
       lists[kMSetup].push_back(t);
       lists[lp <= 4 ? kMOrthoA : kMOrthoB].push_back(t);
       if (!tl->olev[tl->task_level[t]].tiled) lists[lp <= 4 ? kMDirectA : kMDirectB].push_back(t);
-    } else if (c == kClsOrthoClassed && wave && lp <= 8) {
-      lists[kMSetupHi].push_back(t);
-      lists[kMOrthoC].push_back(t);
-      if (!tl->olev[tl->task_level[t]].tiled) lists[kMDirectC].push_back(t);
-      lists[kMOrthoRest].push_back(t);  // integrate: per-task kernels
+    } else if (c == kClsOrthoClassed && wave && lp <= 11) {
+      if (lp <= 8) {
+        lists[kMSetupHi].push_back(t);
+        lists[kMOrthoC].push_back(t);
+        if (!tl->olev[tl->task_level[t]].tiled) lists[kMDirectC].push_back(t);
+      } else {
+        lists[kMOrthoRestC].push_back(t);
+      }
+      lists[kMOrthoHl].push_back(t);  // integrate
     } else {
       lists[kMOrthoRest].push_back(t);
       lists[kMOrthoRestC].push_back(t);
@@ -1702,9 +1713,19 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
                                    : launch_general_collocate(R, lpb, s));
     }
     if (mode == kIntegrate) {
+      PointParams H = sub(kMOrthoHl);
+      if (H.ntasks > 0) {
+        H.wp = make_wave_plan(tl, -1, std::min(11, mp.lp[kMOrthoHl]), P.ld, kPlanMfma);
+        TimedLaunch tm(9, s);
+        HIP_CHECK(launch_integrate_mfma(H, s));
+      }
       TimedLaunch tm(10, s);
       Q.lane_cxyz = 0;
       HIP_CHECK(launch_finish_wave(Q, s));
+      if (H.ntasks > 0) {
+        H.wp = make_wave_plan(tl, -1, std::min(11, mp.lp[kMOrthoHl]), P.ld, kPlanSetup);
+        HIP_CHECK(launch_finish_wave(H, s));
+      }
     }
   }
   for (int m : {mode == kIntegrate ? kMOrthoRest : kMOrthoRestC, kMGenRest}) {

@@ -592,6 +592,99 @@ __device__ __forceinline__ void lane_cxyz(const PointParams &P, const Task &T, c
   for (int e = 0; e < NC; e++) out[e] = acc[e];
 }
 
+// ---------------------------------------------------------------------------
+// Integrate for lp 7..11 on the matrix cores.  Per task the coefficients
+//   cxyz[lx][(ly, lz)] = sum over columns c of px[c][lx] * (py[c][ly] M[c][lz])
+// (M = the column's z moments) are a (lp+1) x NQ x ncol product; the
+// 120..364 accumulators do not fit in lane registers, but on the matrix
+// cores they are 4 x NB doubles per lane.  Columns are walked 64 at a time
+// (one per lane, as in integrate_group), px, py, M staged in LDS, then 16
+// steps of v_mfma_f64_16x16x4f64 over 4 columns each.  Operand layout on
+// gfx950 (checked on the device): A[i][k] and B[k][j] in lane i + 16 k
+// (j + 16 k), D[i][j] in lane j + 16 (i % 4), register i / 4.
+// ---------------------------------------------------------------------------
+typedef double double4_t __attribute__((ext_vector_type(4)));
+constexpr int kMfmaStage = 40;  // LDS doubles per column: px[16] (zero padded), py[12], M[12]
+
+template <int LP>
+__device__ __forceinline__ void integrate_mfma_task(const PointParams &P, const Task &T,
+                                                    const GeoClass &G, const Layout &lay,
+                                                    const WavePlan &wp, const double *tab,
+                                                    double *stage, const double *grid,
+                                                    double *out, int lane) {
+  constexpr int NQ = (LP + 1) * (LP + 2) / 2;  // (ly, lz) with ly + lz <= LP, lz-major
+  constexpr int NB = (NQ + 15) / 16;
+  const int NP = wp.np;
+  const double *tabz = tab + 2 * wp.ext * NP;
+  const int nxy = lay.npts_local[0] * lay.npts_local[1];
+  const int ngz = lay.npts_global[2], nz = lay.npts_local[2];
+  // (ly, lz) of this lane's B column in each block
+  int qy[NB], qz[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++) {
+    int n = 16 * nb + (lane & 15), z = 0;
+    while (z <= LP && n >= LP - z + 1) {
+      n -= LP - z + 1;
+      z++;
+    }
+    qy[nb] = n;
+    qz[nb] = z;  // z > LP: padding column
+  }
+  double4_t acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++) acc[nb] = double4_t{0.0, 0.0, 0.0, 0.0};
+  int cw[3];
+  axis_offsets(T, lay, cw);
+  double *sc = stage + lane * kMfmaStage;
+  for (int c0 = 0; c0 < G.ncol; c0 += 64) {
+    const Column C = load_column(P, G, lay, cw, c0 + lane);
+    const unsigned span = (unsigned)(2 - 2 * C.kl);
+    const int kmin = P.geo_kmin[G.grp_off + (c0 >> 6)];
+    double M[LP + 1];
+#pragma unroll
+    for (int l = 0; l <= LP; l++) M[l] = 0.0;
+    int gz = wrap(cw[2] + kmin, ngz);
+#pragma unroll 2
+    for (int k = kmin; k <= 1 - kmin; k++) {
+      const double *pz = tabz + (k - T.lo[2]) * NP;
+      double g = 0.0;
+      if (gz < nz && (unsigned)(k - C.kl) < span) g = grid[gz * nxy + C.base];
+#pragma unroll
+      for (int l = 0; l <= LP; l++) M[l] = fma(g, pz[l], M[l]);
+      gz = (gz + 1 == ngz) ? 0 : gz + 1;
+    }
+    double px[LP + 1], py[LP + 1];
+    load_pxy<LP>(T, wp, tab, C, px, py);
+#pragma unroll
+    for (int l = 0; l < 16; l++) sc[l] = l <= LP ? px[l < LP ? l : LP] : 0.0;
+#pragma unroll
+    for (int l = 0; l <= LP; l++) {
+      sc[16 + l] = py[l];
+      sc[28 + l] = M[l];
+    }
+    wave_sync();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; kk++) {
+      const double *sk = stage + (4 * kk + (lane >> 4)) * kMfmaStage;
+      const double a = sk[lane & 15];
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        const double b = qz[nb] <= LP ? sk[16 + qy[nb]] * sk[28 + qz[nb]] : 0.0;
+        acc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[nb], 0, 0, 0);
+      }
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int lx = 4 * r + (lane >> 4);
+      if (qz[nb] <= LP && lx <= LP - qy[nb] - qz[nb])
+        out[packed_index(lx, qy[nb], qz[nb], LP)] = acc[nb][r];
+    }
+}
+
 struct TaskL {
   int la, lb, la_min, lb_min, lp;
 };
@@ -803,6 +896,37 @@ __global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel
   }
 }
 
+// Integration of orthorhombic tasks with lp 7..11, one task per wave, the
+// column contraction on the matrix cores (integrate_mfma_task).  FP64 in both
+// precision modes.  LDS per wave: tables at o_u, staging at o_cxyz.
+__global__ __launch_bounds__(256) void integrate_mfma_kernel(const Task *__restrict__ tasks,
+                                                             const Level *__restrict__ levels,
+                                                             PointParams P) {
+  extern __shared__ double lds[];
+  const WavePlan &wp = P.wp;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (t >= P.ntasks) return;
+  const long long task = launch_task(P, t);
+  const Task &T = tasks[task];
+  const GeoClass &G = P.geo[T.geo];
+  const Level &L = levels[T.level];
+  const Layout &lay = L.lay;
+  const WaveLds W = wave_lds(lds, wp, wave);
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  wave_tables(T, lay, lp, wp, W.u, lane);
+  double *out = P.cxyz_all + task * P.cxyz_stride;
+  switch (lp) {
+    case 7: integrate_mfma_task<7>(P, T, G, lay, wp, W.u, W.cxyz, L.grid, out, lane); break;
+    case 8: integrate_mfma_task<8>(P, T, G, lay, wp, W.u, W.cxyz, L.grid, out, lane); break;
+    case 9: integrate_mfma_task<9>(P, T, G, lay, wp, W.u, W.cxyz, L.grid, out, lane); break;
+    case 10: integrate_mfma_task<10>(P, T, G, lay, wp, W.u, W.cxyz, L.grid, out, lane); break;
+    case 11: integrate_mfma_task<11>(P, T, G, lay, wp, W.u, W.cxyz, L.grid, out, lane); break;
+    default: break;  // the host routes other lp elsewhere
+  }
+}
+
 // Packed cxyz of one task per wave -> hab, forces, virial.
 template <bool FULL>
 __global__ __launch_bounds__(256, (FULL ? 2 : 4)) void finish_wave_kernel(
@@ -902,6 +1026,13 @@ hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
   return P.fp32 ? integrate_wave_r<float>(P, lpb, s) : integrate_wave_r<double>(P, lpb, s);
+}
+
+hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0 || P.wp.np > 12)
+    return hipErrorInvalidValue;
+  return launch_wave(integrate_mfma_kernel, P, wave_blocks(P), s, P.tasks, P.levels);
 }
 
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s) {
