@@ -51,6 +51,7 @@ hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hip
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
 hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s);
+hipError_t launch_finish_block(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
@@ -1264,7 +1265,8 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
     int n_setup = 0;
     for (int la = 0; la <= tl->max_la + ld[0]; la++)
       for (int lb = 0; lb <= tl->max_lb + ld[2] && la + lb <= lp; lb++)
-        n_setup = std::max(n_setup, ncoset(la) * ncoset(lb) + 3 * (la + 1) * (lb + 1) * w.np);
+        n_setup = std::max(n_setup, ncoset(la) * ncoset(lb) + 3 * (la + 1) * (lb + 1) * w.np +
+                                        (la + 1) * (lb + 1) * w.np * w.np);  // + x sums of the finish
     w.o_cxyz = 0;
     w.o_u = w.np * w.np * w.np;
     nd = w.o_u + n_setup;
@@ -1432,9 +1434,21 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       if (mode == kIntegrate) {
         TimedLaunch tm(10, s);
         // one wave per task: its hab atomics are contiguous per task (a
-        // task per lane scatters them over 64 blocks and measured slower)
+        // task per lane scatters them over 64 blocks and measured slower);
+        // lp 7..11: a workgroup per task (the tables fill most of LDS)
         Q.lane_cxyz = 0;
-        HIP_CHECK(launch_finish_wave(Q, s));
+        if (c2 > o.begin) {
+          PointParams F = Q;
+          F.ntasks = c2 - o.begin;
+          F.wp = make_wave_plan(tl, l, std::min(6, tl->lp0s[c2 - 1] + sh), P.ld, kPlanSetup);
+          HIP_CHECK(launch_finish_wave(F, s));
+        }
+        if (c3 > c2) {
+          PointParams F = Q;
+          F.task_base = c2;
+          F.ntasks = c3 - c2;
+          HIP_CHECK(launch_finish_block(F, s));
+        }
       }
     }
     if (o.end > c3) {
@@ -1724,7 +1738,7 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       HIP_CHECK(launch_finish_wave(Q, s));
       if (H.ntasks > 0) {
         H.wp = make_wave_plan(tl, -1, std::min(11, mp.lp[kMOrthoHl]), P.ld, kPlanSetup);
-        HIP_CHECK(launch_finish_wave(H, s));
+        HIP_CHECK(launch_finish_block(H, s));
       }
     }
   }

@@ -425,34 +425,62 @@ __device__ __forceinline__ void integrate_task(const PointParams &P, const Task 
 
 // cxyz -> cab -> hab (Cartesian, atomics into the group's slice), forces and
 // virial per task (grid_ref_integrate.c:132-171).
-template <bool FULL>
-__device__ void wave_finish(const PointParams &P, const Task &T, const Group &G, int la, int lb,
-                            int la_min, int lb_min, const WavePlan &wp, const WaveLds &W,
-                            long long task, int lane) {
-  const int NP = wp.np;
+// NT threads per task: 64 (one wave, wave_sync) or a whole workgroup
+// (__syncthreads; red: LDS for the per-wave force/virial partial sums).
+template <int NT>
+__device__ __forceinline__ void task_sync() {
+  if constexpr (NT == 64)
+    wave_sync();
+  else
+    __syncthreads();
+}
+
+template <bool FULL, int NT>
+__device__ void task_finish(const PointParams &P, const Task &T, const Group &G, int la, int lb,
+                            int la_min, int lb_min, const WaveLds &W, double *red,
+                            long long task, int tid) {
+  const int lane = tid & 63;
   const int n1 = ncoset(la), n2 = ncoset(lb);
-  double *cab = W.u, *alpha = W.u + n1 * n2;
-  build_alpha(T, la, lb, alpha, lane, 64);
-  wave_sync();
-  const int np = la + lb + 1, nA = la + 1, nB = lb + 1;
+  const int np = la + lb + 1, nA = la + 1, nB = lb + 1, lp = la + lb;
+  double *cab = W.u, *alpha = W.u + n1 * n2, *X = alpha + 3 * nA * nB * np;
+  build_alpha(T, la, lb, alpha, tid, NT);
+  task_sync<NT>();
+  // separable adjoint of cab -> cxyz: first along x for every (ax, bx),
+  //   X[ax][bx][lz][ly] = sum_lx alpha_x[bx][ax][lx] cxyz[lx, ly, lz]
+  // (W.cxyz holds the packed cxyz), then per cab element the y and z sums
+  for (int e = tid; e < nA * nB * np * np; e += NT) {
+    const int ly = e % np, lz = (e / np) % np, ab = e / (np * np);
+    const int ax = ab / nB, bx = ab % nB;
+    double s = 0.0;
+    if (ly + lz <= lp) {
+      const double *al = &alpha[(bx * nA + ax) * np];
+      const double *c = W.cxyz + packed_index(0, ly, lz, lp);
+      const int top = min(ax + bx, lp - ly - lz);
+      for (int lx = 0; lx <= top; lx++) s = fma(al[lx], c[lx], s);
+    }
+    X[e] = s;
+  }
+  task_sync<NT>();
   const int a1 = ncoset(la_min - 1), ma = ncoset(la) - a1;
   const int b1 = ncoset(lb_min - 1), mb = ncoset(lb) - b1;
-  for (int e = lane; e < n1 * n2; e += 64) {
+  for (int e = tid; e < n1 * n2; e += NT) {
     const int ia = e % n1, jb = e / n1;
     double s = 0.0;
     if (ia >= a1 && ia < a1 + ma && jb >= b1 && jb < b1 + mb) {
       const Ang A = orb(ia), B = orb(jb);
-      const double *ax = &alpha[((0 * nB + B.x) * nA + A.x) * np];
       const double *ay = &alpha[((1 * nB + B.y) * nA + A.y) * np];
       const double *az = &alpha[((2 * nB + B.z) * nA + A.z) * np];
-      for (int lz = 0; lz <= A.z + B.z; lz++)
-        for (int ly = 0; ly <= A.y + B.y; ly++)
-          for (int lx = 0; lx <= A.x + B.x; lx++)
-            s += W.cxyz[(lz * NP + ly) * NP + lx] * (ax[lx] * ay[ly] * az[lz] * T.pexp);
+      const double *x = X + (A.x * nB + B.x) * np * np;
+      for (int lz = 0; lz <= A.z + B.z; lz++) {
+        double t = 0.0;
+        for (int ly = 0; ly <= A.y + B.y; ly++) t = fma(ay[ly], x[lz * np + ly], t);
+        s = fma(az[lz], t, s);
+      }
+      s *= T.pexp;
     }
     cab[e] = s;
   }
-  wave_sync();
+  task_sync<NT>();
   const CabGet cg{cab, n1};
   const bool tau = FULL && P.compute_tau != 0;
   const double za = T.zeta, zb = T.zetb;
@@ -460,7 +488,7 @@ __device__ void wave_finish(const PointParams &P, const Task &T, const Group &G,
   const int b0 = ncoset(T.lb_min - 1), nb = ncoset(T.lb_max) - b0;
   double f[6] = {0, 0, 0, 0, 0, 0};
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int e = lane; e < na * nb; e += 64) {
+  for (int e = tid; e < na * nb; e += NT) {
     const int ia = a0 + e % na, jb = b0 + e / na;
     const Ang A = orb(ia), B = orb(jb);
     const long long q = G.cart_off + (long long)(T.o2 + jb) * G.ncoa + T.o1 + ia;
@@ -485,7 +513,16 @@ __device__ void wave_finish(const PointParams &P, const Task &T, const Group &G,
     const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
     for (int c = 0; c < kTaskOutStride; c++) {
       const double s = wave_total(c < 6 ? f[c] : v[c - 6]);
-      if (lane == 63) out[c] = scale * s;
+      if (NT == 64 && lane == 63) out[c] = scale * s;
+      if (NT > 64 && lane == 63) red[(tid >> 6) * kTaskOutStride + c] = s;
+    }
+    if constexpr (NT > 64) {
+      __syncthreads();
+      if (tid < kTaskOutStride) {
+        double s = 0.0;
+        for (int w = 0; w < NT / 64; w++) s += red[w * kTaskOutStride + tid];
+        out[tid] = scale * s;
+      }
     }
   }
 }
@@ -940,14 +977,32 @@ __global__ __launch_bounds__(256, (FULL ? 2 : 4)) void finish_wave_kernel(
   const Task &T = tasks[task];
   const TaskL l = task_l(P, T);
   const WaveLds W = wave_lds(lds, P.wp, wave);
-  const int NP = P.wp.np;
   const double *in = P.cxyz_all + task * P.cxyz_stride;
-  for (int e = lane; e < NP * NP * NP; e += 64) {
-    const int lz = e / (NP * NP), ly = (e / NP) % NP, lx = e % NP;
-    W.cxyz[e] = (lx + ly + lz <= l.lp) ? in[packed_index(lx, ly, lz, l.lp)] : 0.0;
-  }
+  const int nc = (l.lp + 1) * (l.lp + 2) * (l.lp + 3) / 6;
+  for (int e = lane; e < nc; e += 64) W.cxyz[e] = in[e];  // packed
   wave_sync();
-  wave_finish<FULL>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, P.wp, W, task, lane);
+  task_finish<FULL, 64>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, W, nullptr, task,
+                        lane);
+}
+
+// The same per task with a whole workgroup (256 threads) on one task: for
+// high l, where one task's tables fill most of LDS.
+template <bool FULL>
+__global__ __launch_bounds__(256) void finish_block_kernel(const Task *__restrict__ tasks,
+                                                           const Group *__restrict__ groups,
+                                                           PointParams P) {
+  extern __shared__ double lds[];
+  const long long task = launch_task(P, blockIdx.x);
+  const Task &T = tasks[task];
+  const TaskL l = task_l(P, T);
+  const WaveLds W = wave_lds(lds, P.wp, 0);
+  double *red = lds + P.wp.wave_bytes / 8;
+  const double *in = P.cxyz_all + task * P.cxyz_stride;
+  const int nc = (l.lp + 1) * (l.lp + 2) * (l.lp + 3) / 6;
+  for (int e = threadIdx.x; e < nc; e += 256) W.cxyz[e] = in[e];  // packed
+  __syncthreads();
+  task_finish<FULL, 256>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, W, red, task,
+                         threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1033,6 +1088,24 @@ hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s) {
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0 || P.wp.np > 12)
     return hipErrorInvalidValue;
   return launch_wave(integrate_mfma_kernel, P, wave_blocks(P), s, P.tasks, P.levels);
+}
+
+// One workgroup per task; P.wp: the setup plan (one region of wave_bytes).
+hipError_t launch_finish_block(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  if (P.wp.window != 0) return hipErrorInvalidValue;
+  const bool full = P.do_forces || P.compute_tau;
+  const size_t lds = (size_t)P.wp.wave_bytes + 8 * 4 * kTaskOutStride;
+  auto go = [&](auto kernel) {
+    if (lds > 65536) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kernel, dim3(P.ntasks), dim3(256), lds, s, P.tasks, P.groups, P);
+    return hipGetLastError();
+  };
+  return full ? go(finish_block_kernel<true>) : go(finish_block_kernel<false>);
 }
 
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s) {
