@@ -1250,7 +1250,7 @@ enum LaunchMode { kCollocate = 0, kIntegrate = 1, kCount = 2 };
 // with lp <= `lp`.  Setup/finish kernels hold dense cxyz and {cab, alpha} per
 // wave; point kernels hold the per-axis tables, and collocation keeps the
 // whole level grid in LDS when it fits next to at least four wave regions.
-enum PlanKind { kPlanSetup, kPlanPoints, kPlanWindow, kPlanMfma };
+enum PlanKind { kPlanSetup, kPlanPoints, kPlanWindow, kPlanMfma, kPlanFinish };
 
 // level -1: a merged launch over all levels (not for kPlanWindow).
 WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[4], int kind) {
@@ -1260,15 +1260,17 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
   w.np = lp + 1;
   w.ext = level < 0 ? tl->max_ext_all : o.max_ext;
   int nd;
-  if (kind == kPlanSetup) {
-    // cab [ncoset(la)][ncoset(lb)] + alpha [3][lb+1][la+1][lp+1] for la + lb <= lp
+  if (kind == kPlanSetup || kind == kPlanFinish) {
+    // cab [ncoset(la)][ncoset(lb)] + alpha [3][lb+1][la+1][lp+1] for la + lb <= lp;
+    // the finish adds its x sums [la+1][lb+1][(lp+1)(lp+2)/2] and keeps cxyz
+    // packed, the setup keeps cxyz dense [lp+1]^3
     int n_setup = 0;
     for (int la = 0; la <= tl->max_la + ld[0]; la++)
       for (int lb = 0; lb <= tl->max_lb + ld[2] && la + lb <= lp; lb++)
         n_setup = std::max(n_setup, ncoset(la) * ncoset(lb) + 3 * (la + 1) * (lb + 1) * w.np +
-                                        (la + 1) * (lb + 1) * w.np * w.np);  // + x sums of the finish
+                                        (kind == kPlanFinish ? (la + 1) * (lb + 1) * w.np * (w.np + 1) / 2 : 0));
     w.o_cxyz = 0;
-    w.o_u = w.np * w.np * w.np;
+    w.o_u = kind == kPlanFinish ? (lp + 1) * (lp + 2) * (lp + 3) / 6 : w.np * w.np * w.np;
     nd = w.o_u + n_setup;
   } else if (kind == kPlanMfma) {
     // per-axis tables, then the staging of 64 columns (40 doubles each)
@@ -1440,13 +1442,14 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         if (c2 > o.begin) {
           PointParams F = Q;
           F.ntasks = c2 - o.begin;
-          F.wp = make_wave_plan(tl, l, std::min(6, tl->lp0s[c2 - 1] + sh), P.ld, kPlanSetup);
+          F.wp = make_wave_plan(tl, l, std::min(6, tl->lp0s[c2 - 1] + sh), P.ld, kPlanFinish);
           HIP_CHECK(launch_finish_wave(F, s));
         }
         if (c3 > c2) {
           PointParams F = Q;
           F.task_base = c2;
           F.ntasks = c3 - c2;
+          F.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanFinish);
           HIP_CHECK(launch_finish_block(F, s));
         }
       }
@@ -1569,6 +1572,7 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       if (mode == kIntegrate) {
         TimedLaunch tm(10, s);
         Q.lane_cxyz = 0;
+        Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanFinish);
         HIP_CHECK(launch_finish_wave(Q, s));
       }
     }
@@ -1735,9 +1739,10 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       }
       TimedLaunch tm(10, s);
       Q.lane_cxyz = 0;
+      Q.wp = make_wave_plan(tl, -1, std::min(6, mp.lp[kMSetup]), P.ld, kPlanFinish);
       HIP_CHECK(launch_finish_wave(Q, s));
       if (H.ntasks > 0) {
-        H.wp = make_wave_plan(tl, -1, std::min(11, mp.lp[kMOrthoHl]), P.ld, kPlanSetup);
+        H.wp = make_wave_plan(tl, -1, std::min(11, mp.lp[kMOrthoHl]), P.ld, kPlanFinish);
         HIP_CHECK(launch_finish_block(H, s));
       }
     }

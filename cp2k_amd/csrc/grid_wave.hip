@@ -446,18 +446,23 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
   build_alpha(T, la, lb, alpha, tid, NT);
   task_sync<NT>();
   // separable adjoint of cab -> cxyz: first along x for every (ax, bx),
-  //   X[ax][bx][lz][ly] = sum_lx alpha_x[bx][ax][lx] cxyz[lx, ly, lz]
-  // (W.cxyz holds the packed cxyz), then per cab element the y and z sums
-  for (int e = tid; e < nA * nB * np * np; e += NT) {
-    const int ly = e % np, lz = (e / np) % np, ab = e / (np * np);
-    const int ax = ab / nB, bx = ab % nB;
-    double s = 0.0;
-    if (ly + lz <= lp) {
-      const double *al = &alpha[(bx * nA + ax) * np];
-      const double *c = W.cxyz + packed_index(0, ly, lz, lp);
-      const int top = min(ax + bx, lp - ly - lz);
-      for (int lx = 0; lx <= top; lx++) s = fma(al[lx], c[lx], s);
+  //   X[ax][bx][(lz, ly)] = sum_lx alpha_x[bx][ax][lx] cxyz[lx, ly, lz],
+  // (lz, ly) packed with ly + lz <= lp (row lz holds np - lz entries; W.cxyz
+  // holds the packed cxyz), then per cab element the y and z sums
+  const int nq = np * (np + 1) / 2;
+  for (int e = tid; e < nA * nB * nq; e += NT) {
+    int q = e % nq, lz = 0;
+    const int ab = e / nq, ax = ab / nB, bx = ab % nB;
+    while (q >= np - lz) {
+      q -= np - lz;
+      lz++;
     }
+    const int ly = q;
+    const double *al = &alpha[(bx * nA + ax) * np];
+    const double *c = W.cxyz + packed_index(0, ly, lz, lp);
+    const int top = min(ax + bx, lp - ly - lz);
+    double s = 0.0;
+    for (int lx = 0; lx <= top; lx++) s = fma(al[lx], c[lx], s);
     X[e] = s;
   }
   task_sync<NT>();
@@ -470,10 +475,11 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
       const Ang A = orb(ia), B = orb(jb);
       const double *ay = &alpha[((1 * nB + B.y) * nA + A.y) * np];
       const double *az = &alpha[((2 * nB + B.z) * nA + A.z) * np];
-      const double *x = X + (A.x * nB + B.x) * np * np;
+      const double *x = X + (A.x * nB + B.x) * nq;
       for (int lz = 0; lz <= A.z + B.z; lz++) {
+        const double *xr = x + lz * np - lz * (lz - 1) / 2;  // row lz of the packed (lz, ly)
         double t = 0.0;
-        for (int ly = 0; ly <= A.y + B.y; ly++) t = fma(ay[ly], x[lz * np + ly], t);
+        for (int ly = 0; ly <= A.y + B.y; ly++) t = fma(ay[ly], xr[ly], t);
         s = fma(az[lz], t, s);
       }
       s *= T.pexp;
