@@ -604,7 +604,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     if (2.0 * radius_list[t] < level_hmax[level_list[t] - 1])
       cls[t] = 2;
     else
-      cls[t] = (orthorhombic && border_mask_list[t] == 0) ? 0 : 1;
+      cls[t] = ((level_ortho ? level_ortho[level_list[t] - 1] != 0 : orthorhombic) &&
+                border_mask_list[t] == 0) ? 0 : 1;  // the task's kernel class, as below
   }
   std::vector<int> order(ntasks);
   std::iota(order.begin(), order.end(), 0);

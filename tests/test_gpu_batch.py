@@ -1,7 +1,8 @@
 """Batched task lists (grid_mi355x_create_batch / grid_mi355x_batch_run).
 
 A batch of lists with different cells, grid sizes and level counts (cubic
-one-level lists, a multi-level water box, a triclinic list) must give each
+one-level lists, a multi-level water box, a triclinic list, an l <= 4 list
+on five levels) must give each
 list exactly what the list gives on its own (summation order aside, 1e-12
 as in the FP64 parity tests).  A run replays a captured hipGraph: every
 replay is checked, with eager calls and synchronisations in between.
@@ -27,6 +28,8 @@ def lists():
     out = synth.batch_config(4, seed=5, atoms=(8, 20))
     out.append(synth.config("h2o2", seed=7, n=32))
     out.append(synth.config("tri10", seed=8))
+    # la, lb <= 4: lp up to 8 in collocate and 11 with forces and virial
+    out.append(synth.config("hl2048", seed=9, keep=6))
     return out
 
 
