@@ -79,6 +79,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-fp32", action="store_true",
                    help="skip the FP32-mixed leg (throughput and per-result errors)")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep the default tile launch shapes (no grid_mi355x_autotune)")
     p.add_argument("--secondary", default="tri1024",
                    help="comma-separated workloads also measured (FP64 and FP32-mixed, "
                         "no CPU leg) and reported under 'secondary'; empty for none")
@@ -392,6 +394,15 @@ def measure(opt, name, cpu):
         gt.integrate_dev(V_d, hab_d, pab=pab_d if opt.forces else None, forces=f_d,
                          virial=v_d)
 
+    tuned = None
+    if not opt.no_autotune:
+        t0 = time.time()
+        gt.autotune(100, pab_d, rho_d)
+        tuned = {"seconds": round(time.time() - t0, 2), "levels": [
+            {"ortho": gt.tuned_shape(l), "general": gt.tuned_shape(l, general=True)}
+            for l in range(args.nlevels)]}
+        log(f"{name}: autotuned in {tuned['seconds']} s")
+
     if opt.copy_calibration:
         n = opt.copy_calibration * (1 << 20) // 8
         src, dst = grid.DeviceArray((n,)), grid.DeviceArray((n,))
@@ -432,6 +443,7 @@ def measure(opt, name, cpu):
             "forces": bool(opt.forces),
             "tasks_per_s": round(2 * args.ntasks / (ms_step * 1e-3), 1),
             "generate_s": round(gen_s, 2),
+            "autotune": tuned,
         },
         "kernel_ms_per_step": {k: round(v[0] / max(opt.steps, 1), 4) for k, v in kt.items()},
         "roofline": roofline(opt, kt, by_lp, lambda dom: algorithmic_bytes(args, points, dom),

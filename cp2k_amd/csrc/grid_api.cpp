@@ -318,6 +318,9 @@ struct OrthoLevel {
   // collocation tiles (classed levels): nx * by * bz points
   bool tiled = false;
   int by = 0, bz = 0;
+  // launch shape of the tile kernels (grid_mi355x_autotune): workgroup
+  // rounds per CU over the level's items, largest waves per workgroup
+  int rounds = 4, max_waves = 12;
 };
 
 // Items of one tile, sorted by lp0: start[v] = first item with lp0 >= v.
@@ -1295,7 +1298,8 @@ WavePlan make_wave_plan(const TaskListImpl *tl, int level, int lp, const int ld[
         if (tl->olev[l].tiled)
           npts = std::max(npts, (long long)tl->npts_local[3 * l] * tl->olev[l].by * tl->olev[l].bz);
     }
-    const int most = lp <= 4 ? 12 : 8;  // launch bounds of the tile kernels
+    // launch bounds of the tile kernels, and the tuned cap of the level
+    const int most = std::min(lp <= 4 ? 12 : 8, level >= 0 ? o.max_waves : 12);
     for (int nw : {12, 10, 8, 6, 4})
       if (nw <= most && 8 * npts + (long long)nw * w.wave_bytes <= limit) {
         w.window = (int)npts;
@@ -1352,8 +1356,8 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false) {
         if (t.level == l) total += t.start[v1] - t.start[v0];
       const int first = (int)units.size();
       if (total > 0) {
-        const long long per =
-            std::max<long long>(32, (total + 4LL * num_cus() - 1) / (4LL * num_cus()));
+        const long long rounds = (long long)lev[l].rounds * num_cus();
+        const long long per = std::max<long long>(32, (total + rounds - 1) / rounds);
         for (const TileInfo &t : tiles) {
           if (t.level != l) continue;
           const int a = t.start[v0], b = t.start[v1];
@@ -1482,6 +1486,7 @@ WavePlan general_plan(int lp) {
 // level): the largest tile (row stride nx | 1) and wave regions holding the
 // packed cxyz and the dense cijk.
 bool general_tile_plan(const TaskListImpl *tl, int level, int lp, WavePlan &w) {
+  const int cap = level >= 0 ? tl->glev[level].max_waves : 12;
   w = WavePlan{};
   w.level = level;
   w.np = lp + 1;
@@ -1493,7 +1498,7 @@ bool general_tile_plan(const TaskListImpl *tl, int level, int lp, WavePlan &w) {
     if ((level < 0 || l == level) && tl->glev[l].tiled)
       win = std::max(win, (long long)(tl->npts_local[3 * l] | 1) * tl->glev[l].by * tl->glev[l].bz);
   if (win <= 0) return false;
-  const int most = lp <= 4 ? 12 : 8;
+  const int most = std::min(lp <= 4 ? 12 : 8, cap);
   for (int nw : {12, 10, 8, 6, 4})
     if (nw <= most && 8 * win + (long long)nw * w.wave_bytes <= lds_limit()) {
       w.window = (int)win;
@@ -2730,6 +2735,70 @@ void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what, const enum 
 void grid_mi355x_batch_task_points(const grid_mi355x_batch *batch, long long *points) {
   const BatchImpl *b = checked_batch(batch);
   *points = grid_mi355x_task_points((const grid_task_list *)b->merged);
+}
+
+
+// ---- launch-shape autotuner (no reference counterpart) ----
+void grid_mi355x_autotune(const grid_task_list *task_list, const enum grid_func func,
+                          const double *pab_blocks, double *const *grids) {
+  TaskListImpl *tl = checked(task_list);
+  if (tl->empty || tl->merged) return;
+  const hipStream_t s = stream();
+  hipEvent_t a, b;
+  HIP_CHECK(hipEventCreate(&a));
+  HIP_CHECK(hipEventCreate(&b));
+  auto clear_plans = [&]() {
+    sync_stream(s);
+    for (auto &kv : tl->tile_plans) kv.second.d.release();
+    tl->tile_plans.clear();
+    for (auto &kv : tl->gtile_plans) kv.second.d.release();
+    tl->gtile_plans.clear();
+  };
+  auto call_ms = [&]() {
+    run_collocate(tl, func, pab_blocks, grids, s);  // plans and buffers
+    HIP_CHECK(hipEventRecord(a, s));
+    for (int r = 0; r < 3; r++) run_collocate(tl, func, pab_blocks, grids, s);
+    HIP_CHECK(hipEventRecord(b, s));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms / 3.0f;
+  };
+  // coordinate descent, one tiled level at a time, over (rounds, max_waves)
+  for (auto *lv : {&tl->olev, &tl->glev})
+    for (OrthoLevel &o : *lv) {
+      if (o.end <= o.begin || !o.tiled) continue;
+      int best_r = o.rounds, best_w = o.max_waves;
+      clear_plans();
+      float best = call_ms();
+      for (int r : {2, 4, 8})
+        for (int w : {12, 8, 6}) {
+          if (r == best_r && w == best_w) continue;
+          o.rounds = r;
+          o.max_waves = w;
+          clear_plans();
+          const float ms = call_ms();
+          if (ms < best) {
+            best = ms;
+            best_r = r;
+            best_w = w;
+          }
+        }
+      o.rounds = best_r;
+      o.max_waves = best_w;
+      clear_plans();
+    }
+  HIP_CHECK(hipEventDestroy(a));
+  HIP_CHECK(hipEventDestroy(b));
+}
+
+void grid_mi355x_tuned_shape(const grid_task_list *task_list, const int level, const int general,
+                             int *rounds, int *max_waves) {
+  const TaskListImpl *tl = checked(task_list);
+  if (level < 0 || level >= tl->nlevels) fatal("level %d out of range", level);
+  const OrthoLevel &o = general ? tl->glev[level] : tl->olev[level];
+  *rounds = o.rounds;
+  *max_waves = o.max_waves;
 }
 
 }  // extern "C"

@@ -36,6 +36,7 @@ EXPORTS = [
     "grid_mi355x_task_points_by_lp", "grid_mi355x_task_counts",
     "grid_mi355x_create_batch", "grid_mi355x_free_batch", "grid_mi355x_batch_buffers",
     "grid_mi355x_batch_grid", "grid_mi355x_batch_run", "grid_mi355x_batch_task_points",
+    "grid_mi355x_autotune", "grid_mi355x_tuned_shape",
 ]
 
 GRID_FUNC_AB = 100
@@ -309,6 +310,18 @@ class GridTaskList:
 
     def task_points(self):
         return int(lib().grid_mi355x_task_points(self.h))
+
+    def autotune(self, func, pab, grids):
+        """grid_mi355x_autotune on device buffers (DeviceArray or views)."""
+        a = self.args
+        gp = (C.c_void_p * a.nlevels)(*[g.data_ptr() for g in grids])
+        lib().grid_mi355x_autotune(self.h, C.c_int(func), C.c_void_p(pab.data_ptr()), gp)
+
+    def tuned_shape(self, level, general=False):
+        r, w = C.c_int(), C.c_int()
+        lib().grid_mi355x_tuned_shape(self.h, C.c_int(level), C.c_int(1 if general else 0),
+                                      C.byref(r), C.byref(w))
+        return r.value, w.value
 
     def task_points_by_lp(self, n=11):
         """Task points per la_max + lb_max of the tasks (index = lp)."""

@@ -149,6 +149,18 @@ void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what,
 /* Task points of all lists of the batch. */
 void grid_mi355x_batch_task_points(const grid_mi355x_batch *batch, long long *points);
 
+/* Launch-shape autotuner: for every level that collocates through LDS tiles,
+ * times the collocation (func, pab_blocks and grids as in
+ * grid_mi355x_collocate_task_list_dev) for each workgroup count (2, 4 or 8
+ * rounds per CU over the level's tile items) and cap of waves per workgroup
+ * (12, 8, 6), one level after the other, and keeps the fastest in the list
+ * (later collocations use it).  Synchronous.  tuned_shape reads the choice of
+ * a level (general = 1: the general-cell tiles of the level). */
+void grid_mi355x_autotune(const grid_task_list *task_list, const enum grid_func func,
+                          const double *pab_blocks, double *const *grids);
+void grid_mi355x_tuned_shape(const grid_task_list *task_list, const int level,
+                             const int general, int *rounds, int *max_waves);
+
 #ifdef __cplusplus
 }
 #endif

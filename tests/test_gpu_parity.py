@@ -159,6 +159,19 @@ def test_device_calls_in_flight():
     assert replay.rel_err(hab_d.numpy(), want_hab) < 1e-14
 
 
+def test_autotune_keeps_results():
+    args, pab, V = synth.config("h2o64", seed=3, keep=4)
+    gt = grid.GridTaskList(args)
+    pab_d = grid.DeviceArray.from_numpy(pab)
+    rho_d = [grid.DeviceArray(s) for s in args.grid_shapes()]
+    gt.autotune(100, pab_d, rho_d)
+    for level in range(args.nlevels):
+        rounds, waves = gt.tuned_shape(level)
+        assert rounds in (2, 4, 8) and waves in (6, 8, 12)
+    for got, want in zip(gt.collocate(100, pab), O.TaskList(args).collocate(100, pab)):
+        assert replay.rel_err(got, want) < TOL
+
+
 def test_task_points_match_oracle():
     args, pab, V = synth.config("h2o2", seed=4, n=32)
     assert grid.GridTaskList(args).task_points() == O.TaskList(args).task_points()
