@@ -514,4 +514,59 @@ static __device__ __forceinline__ double fmar(double a, double b, double c) { re
 static __device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 
+
+// Sums over the 64 lanes of N per-lane values v[0..N-1], stored to out[e]
+// (each entry by one lane).  Recursive halving: at each of the six steps a
+// lane keeps half of its entries (the lower or upper half, by one bit of its
+// lane id), adds its partner's copy of them and sends the other half: about
+// N exchanges and N additions in all, against 6 N of each for N separate
+// wave_total calls.  Partners: row_mirror (lane bit 3), row_half_mirror
+// (bit 2), quad_perm xor 2, xor 1 (all DPP), then xor 16 and xor 32.  The
+// mirrors also flip the lower bits, which is why they come first: a step's
+// partners must hold the same entries, i.e. agree in the bits used so far.
+template <int S>
+static __device__ __forceinline__ double halving_partner(double v) {
+  if constexpr (S == 0) return dpp_mov<0x140, 0xf>(v);
+  else if constexpr (S == 1) return dpp_mov<0x141, 0xf>(v);
+  else if constexpr (S == 2) return dpp_mov<0x4e, 0xf>(v);
+  else if constexpr (S == 3) return dpp_mov<0xb1, 0xf>(v);
+  else if constexpr (S == 4) return __shfl_xor(v, 16, 64);
+  else return __shfl_xor(v, 32, 64);
+}
+
+template <int N, int S>
+struct HalvingSum {
+  static constexpr int M = (N + 1) / 2;  // entries kept at this step
+  static constexpr int kBit = S == 0 ? 8 : S == 1 ? 4 : S == 2 ? 2 : S == 3 ? 1 : S == 4 ? 16 : 32;
+  // v holds N entries of which the first cnt are real (the rest padding);
+  // they are the sums' entries base .. base + cnt - 1.
+  static __device__ __forceinline__ void run(const double *v, int lane, int base, int cnt,
+                                             double *out) {
+    const bool hi = (lane & kBit) != 0;
+    double w[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      const double lo_v = v[i];
+      const double hi_v = (M + i < N) ? v[(M + i < N) ? M + i : 0] : 0.0;
+      const double keep = hi ? hi_v : lo_v;
+      const double send = hi ? lo_v : hi_v;
+      w[i] = keep + halving_partner<S>(send);
+    }
+    const int b = base + (hi ? M : 0);
+    const int c = hi ? max(cnt - M, 0) : min(cnt, M);
+    if constexpr (S < 5) {
+      HalvingSum<M, S + 1>::run(w, lane, b, c, out);
+    } else {
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        if (i < c) out[b + i] = w[i];
+    }
+  }
+};
+
+template <int N>
+__device__ __forceinline__ void wave_sum_store(const double (&v)[N], double *out, int lane) {
+  HalvingSum<N, 0>::run(v, lane, 0, N, out);
+}
+
 }  // namespace gm

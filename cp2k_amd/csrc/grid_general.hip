@@ -323,11 +323,7 @@ __device__ __forceinline__ void gen_integrate_task(const PointParams &P, const T
 #pragma unroll
   for (int e = 0; e < NC; e++) acc[e] = 0.0;
   gen_integrate_rows<LP, R>(T, lay, grid, acc, lane);
-#pragma unroll
-  for (int e = 0; e < NC; e++) {
-    const double s = wave_total(acc[e]);
-    if (lane == 63) W.cijk[e] = s;  // packed (il, jl, kl)
-  }
+  wave_sum_store<NC>(acc, W.cijk, lane);  // packed (il, jl, kl)
   wave_sync();
   // cxyz[i] = sum_o M[o][i] cijk[o], one output per lane
   const double *M = P.gmat + P.gmat_off[T.level * 8 + LP];

@@ -416,11 +416,7 @@ __device__ __forceinline__ void integrate_task(const PointParams &P, const Task 
 #pragma unroll
   for (int e = 0; e < NC; e++) acc[e] = 0.0;
   integrate_columns<LP, R>(P, T, G, lay, wp, tab, grid, acc, lane);
-#pragma unroll
-  for (int e = 0; e < NC; e++) {
-    const double s = wave_total(acc[e]);
-    if (lane == 63) out[e] = s;
-  }
+  wave_sum_store<NC>(acc, out, lane);
 }
 
 // cxyz -> cab -> hab (Cartesian, atomics into the group's slice), forces and
