@@ -75,6 +75,24 @@ __device__ __forceinline__ void gen_gauss(const Task &T, const Layout &lay, int 
   q = exp(-T.zetp * (2.0 * rh + hh));
 }
 
+// Points i0..i1 of a row as runs a..b of consecutive grid indices gx(a) ..
+// gx(b) inside the border mask: gx(i) = wrap(i - shift, ngx) restarts at 0
+// after ngx - 1, and only blo[0] <= gx <= bhi[0] is kept.  f(a, b, gx(a))
+// per run, so that the per-point loops carry no wrap or mask test.
+template <typename F>
+__device__ __forceinline__ void gen_runs(const Task &T, const Layout &lay, int i0, int i1, F &&f) {
+  const int ngx = lay.npts_global[0];
+  const int top = min(T.bhi[0], ngx - 1);
+  int i = i0, gx = wrap(i0 - lay.shift_local[0], ngx);
+  while (i <= i1) {
+    const int a = i + max(T.blo[0] - gx, 0);
+    const int b = min(i1, i + (top - gx));
+    if (a <= b) f(a, b, gx + (a - i));
+    i += ngx - gx;
+    gx = 0;
+  }
+}
+
 // (a, b, c) of packed index o (inverse of packed_index).
 __device__ __forceinline__ void packed_unpack(int o, int lp, int &a, int &b, int &c) {
   c = 0;
@@ -205,7 +223,6 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
                                               int NP, const TileItem &I, const TileUnit &U,
                                               double *win, int lane) {
   const int nj = I.j1 - I.j0 + 1, nk = I.k1 - I.k0 + 1;
-  const int ngx = lay.npts_global[0];
   const int sx = gen_tile_stride(lay.npts_local[0]);
   double hh = 0.0;
 #pragma unroll
@@ -237,23 +254,25 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
       }
       dkp *= dk;
     }
-    int gx = wrap(i0 - lay.shift_local[0], ngx);
-    for (int s0 = i0; s0 <= i1; s0 += kRestart) {
-      double g, q;
-      gen_gauss(T, lay, s0, j, k, g, q);
-      const int s1 = min(i1, s0 + kRestart - 1);
-      double di = s0 - T.gp[0];
-      for (int i = s0; i <= s1; i++) {
-        double p = ci[LP];
+    gen_runs(T, lay, i0, i1, [&](int a, int b, int gxa) {
+      double *dst = row + gxa;
+      for (int s0 = a; s0 <= b; s0 += kRestart) {
+        double g, q;
+        gen_gauss(T, lay, s0, j, k, g, q);
+        const int n = min(b - s0 + 1, kRestart);
+        double di = s0 - T.gp[0];
+        for (int s = 0; s < n; s++) {
+          double p = ci[LP];
 #pragma unroll
-        for (int il = LP - 1; il >= 0; il--) p = fma(p, di, ci[il]);
-        if (gx >= T.blo[0] && gx <= T.bhi[0]) atomicAdd(&row[gx], p * g);
-        g *= q;
-        q *= r2step;
-        di += 1.0;
-        gx = (gx + 1 == ngx) ? 0 : gx + 1;
+          for (int il = LP - 1; il >= 0; il--) p = fma(p, di, ci[il]);
+          atomicAdd(dst + s, p * g);
+          g *= q;
+          q *= r2step;
+          di += 1.0;
+        }
+        dst += n;
       }
-    }
+    });
   }
 }
 
@@ -263,7 +282,6 @@ template <int LP, typename R>
 __device__ __forceinline__ void gen_integrate_rows(const Task &T, const Layout &lay,
                                                    const R *grid, double *acc, int lane) {
   const int nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
-  const int ngx = lay.npts_global[0];
   double hh = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
@@ -276,26 +294,27 @@ __device__ __forceinline__ void gen_integrate_rows(const Task &T, const Layout &
     double ci[LP + 1];
 #pragma unroll
     for (int il = 0; il <= LP; il++) ci[il] = 0.0;
-    int gx = wrap(i0 - lay.shift_local[0], ngx);
-    for (int s0 = i0; s0 <= i1; s0 += kRestart) {
-      double g, q;
-      gen_gauss(T, lay, s0, j, k, g, q);
-      const int s1 = min(i1, s0 + kRestart - 1);
-      double di = s0 - T.gp[0];
-      for (int i = s0; i <= s1; i++) {
-        double t = 0.0;
-        if (gx >= T.blo[0] && gx <= T.bhi[0]) t = (double)grid[base + gx] * g;
+    gen_runs(T, lay, i0, i1, [&](int a, int b, int gxa) {
+      const R *src = grid + base + gxa;
+      for (int s0 = a; s0 <= b; s0 += kRestart) {
+        double g, q;
+        gen_gauss(T, lay, s0, j, k, g, q);
+        const int n = min(b - s0 + 1, kRestart);
+        double di = s0 - T.gp[0];
+        for (int s = 0; s < n; s++) {
+          double t = (double)src[s] * g;
 #pragma unroll
-        for (int il = 0; il <= LP; il++) {
-          ci[il] += t;
-          t *= di;
+          for (int il = 0; il <= LP; il++) {
+            ci[il] += t;
+            t *= di;
+          }
+          g *= q;
+          q *= r2step;
+          di += 1.0;
         }
-        g *= q;
-        q *= r2step;
-        di += 1.0;
-        gx = (gx + 1 == ngx) ? 0 : gx + 1;
+        src += n;
       }
-    }
+    });
     const double dj = j - T.gp[1], dk = k - T.gp[2];
     double dkp = 1.0;
     int e = 0;
