@@ -53,6 +53,7 @@ hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
 hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s);
 hipError_t launch_finish_block(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
+hipError_t launch_general_cijk(const PointParams &P, hipStream_t s);
 hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 }  // namespace gm
@@ -1557,6 +1558,7 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
         if (!ab || tl->highl_before[c2] > tl->highl_before[g.begin])
           HIP_CHECK(launch_cxyz_wave(Q, s));
+        HIP_CHECK(launch_general_cijk(Q, s));  // the general kernels take cijk
       }
       const int cuts[2][2] = {{g.begin, c1}, {c1, c2}};
       for (int r = 0; r < 2; r++) {
@@ -1671,6 +1673,9 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         H.lane_cxyz = 0;
         HIP_CHECK(launch_cxyz_wave(H, s));
       }
+      PointParams G = sub(kMGenA);  // kMGenA and kMGenB are adjacent: cxyz -> cijk
+      G.ntasks = mp.off[kMGenB + 1] - mp.off[kMGenA];
+      HIP_CHECK(launch_general_cijk(G, s));
     }
     if (mode == kIntegrate) {
       for (int m : {kMOrthoA, kMOrthoB}) {

@@ -93,38 +93,55 @@ __device__ __forceinline__ void gen_runs(const Task &T, const Layout &lay, int i
   }
 }
 
-// (a, b, c) of packed index o (inverse of packed_index).
-__device__ __forceinline__ void packed_unpack(int o, int lp, int &a, int &b, int &c) {
-  c = 0;
-  for (int n = (lp + 1) * (lp + 2) / 2; o >= n; n = (lp - c + 1) * (lp - c + 2) / 2) {
-    o -= n;
-    c++;
+// Rows r = lane, lane + 64, ... of an nj x nk block as offsets (j, k) =
+// (r % nj, r / nj), advanced without a division per row.
+struct RowWalk {
+  int j, k, dj, dk, nj;
+  __device__ __forceinline__ RowWalk(int lane, int nj_) : nj(nj_) {
+    j = lane % nj;
+    k = lane / nj;
+    dj = 64 % nj;
+    dk = 64 / nj;
   }
-  b = 0;
-  for (int n = lp - c + 1; o >= n; n = lp - c - b + 1) {
-    o -= n;
-    b++;
+  __device__ __forceinline__ void next() {
+    j += dj;
+    k += dk;
+    if (j >= nj) {
+      j -= nj;
+      k++;
+    }
   }
-  a = o;
-}
+};
 
-// cxyz (packed, from HBM) -> dense cijk in the wave's LDS through the
-// level's matrix: cijk[o] = sum_i M^T[i][o] cxyz[i], one output per lane.
-__device__ void gen_cijk(const PointParams &P, const Task &T, long long task, int lp,
-                         const GenLds &W, int NP, int lane) {
+// The task's cijk (packed: kl slowest, il fastest, written over its cxyz by
+// general_cijk_kernel) -> the wave's LDS.
+__device__ __forceinline__ void gen_cijk(const PointParams &P, long long task, int lp,
+                                         const GenLds &W, int lane) {
   const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
   const double *src = P.cxyz_all + task * P.cxyz_stride;
-  for (int e = lane; e < nc; e += 64) W.cxyz[e] = src[e];
+  for (int e = lane; e < nc; e += 64) W.cijk[e] = src[e];
   wave_sync();
-  const double *MT = P.gmat + P.gmat_off[T.level * 8 + lp] + nc * nc;
-  for (int o = lane; o < nc; o += 64) {
-    double s = 0.0;
-    for (int i = 0; i < nc; i++) s = fma(MT[i * nc + o], W.cxyz[i], s);
-    int il, jl, kl;
-    packed_unpack(o, lp, il, jl, kl);
-    W.cijk[(kl * NP + jl) * NP + il] = s;
+}
+
+// ci[il] = sum_{jl, kl} cijk[kl][jl][il] dj^jl dk^kl over the packed cijk.
+template <int LP>
+__device__ __forceinline__ void gen_row_poly(const double *cijk, double dj, double dk,
+                                             double *ci) {
+#pragma unroll
+  for (int il = 0; il <= LP; il++) ci[il] = 0.0;
+  double dkp = 1.0;
+  int e = 0;
+#pragma unroll
+  for (int kl = 0; kl <= LP; kl++) {
+    double w = dkp;
+#pragma unroll
+    for (int jl = 0; jl <= LP - kl; jl++) {
+#pragma unroll
+      for (int il = 0; il <= LP - kl - jl; il++, e++) ci[il] = fma(cijk[e], w, ci[il]);
+      w *= dj;
+    }
+    dkp *= dk;
   }
-  wave_sync();
 }
 
 // Collocation staging of one wave: 64 rows x kChunk points, stored lane
@@ -161,24 +178,8 @@ __device__ __forceinline__ void gen_collocate_rows(const Task &T, const Layout &
     long long base = 0;
     if (r < nj * nk && !gen_row(T, lay, j, k, i0, i1, base)) i1 = i0 - 1;
     if (r >= nj * nk) i1 = i0 - 1;
-    // ci[il] = sum_{jl, kl} cijk[kl][jl][il] dj^jl dk^kl
-    const double dj = j - T.gp[1], dk = k - T.gp[2];
     double ci[LP + 1];
-#pragma unroll
-    for (int il = 0; il <= LP; il++) ci[il] = 0.0;
-    double dkp = 1.0;
-#pragma unroll
-    for (int kl = 0; kl <= LP; kl++) {
-      double w = dkp;
-#pragma unroll
-      for (int jl = 0; jl <= LP - kl; jl++) {
-        const double *c = cijk + (kl * NP + jl) * NP;
-#pragma unroll
-        for (int il = 0; il <= LP - kl - jl; il++) ci[il] = fma(c[il], w, ci[il]);
-        w *= dj;
-      }
-      dkp *= dk;
-    }
+    gen_row_poly<LP>(cijk, j - T.gp[1], k - T.gp[2], ci);
     const int len = i1 - i0 + 1;
     const int nch = -wave_min(-((len + kChunk - 1) / kChunk));  // chunks of the longest row
     int gx = wrap(i0 - lay.shift_local[0], ngx);
@@ -228,8 +229,9 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
 #pragma unroll
   for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
   const double r2step = exp(-2.0 * T.zetp * hh);
-  for (int r = lane; r < nj * nk; r += 64) {
-    const int j = I.j0 + r % nj, k = I.k0 + r / nj;
+  RowWalk rw(lane, nj);
+  for (int r = lane; r < nj * nk; r += 64, rw.next()) {
+    const int j = I.j0 + rw.j, k = I.k0 + rw.k;
     int i0, i1;
     long long base;
     if (!gen_row(T, lay, j, k, i0, i1, base)) continue;
@@ -237,23 +239,8 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
     const int lz = wrap(k - lay.shift_local[2], lay.npts_global[2]) - U.gz0;
     if ((unsigned)ly >= (unsigned)U.by || (unsigned)lz >= (unsigned)U.bz) continue;  // not for host-built items
     double *row = win + (lz * U.by + ly) * sx;
-    const double dj = j - T.gp[1], dk = k - T.gp[2];
     double ci[LP + 1];
-#pragma unroll
-    for (int il = 0; il <= LP; il++) ci[il] = 0.0;
-    double dkp = 1.0;
-#pragma unroll
-    for (int kl = 0; kl <= LP; kl++) {
-      double w = dkp;
-#pragma unroll
-      for (int jl = 0; jl <= LP - kl; jl++) {
-        const double *c = cijk + (kl * NP + jl) * NP;
-#pragma unroll
-        for (int il = 0; il <= LP - kl - jl; il++) ci[il] = fma(c[il], w, ci[il]);
-        w *= dj;
-      }
-      dkp *= dk;
-    }
+    gen_row_poly<LP>(cijk, j - T.gp[1], k - T.gp[2], ci);
     gen_runs(T, lay, i0, i1, [&](int a, int b, int gxa) {
       double *dst = row + gxa;
       for (int s0 = a; s0 <= b; s0 += kRestart) {
@@ -286,8 +273,9 @@ __device__ __forceinline__ void gen_integrate_rows(const Task &T, const Layout &
 #pragma unroll
   for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
   const double r2step = exp(-2.0 * T.zetp * hh);
-  for (int r = lane; r < nj * nk; r += 64) {
-    const int j = T.lo[1] + r % nj, k = T.lo[2] + r / nj;
+  RowWalk rw(lane, nj);
+  for (int r = lane; r < nj * nk; r += 64, rw.next()) {
+    const int j = T.lo[1] + rw.j, k = T.lo[2] + rw.k;
     int i0, i1;
     long long base;
     if (!gen_row(T, lay, j, k, i0, i1, base)) continue;
@@ -356,6 +344,34 @@ __device__ __forceinline__ void gen_integrate_task(const PointParams &P, const T
 
 }  // namespace
 
+// cxyz -> cijk of general tasks with lp <= 6, in place (packed, kl slowest,
+// il fastest): cijk[o] = sum_i MT[i][o] cxyz[i] with the level's transposed
+// matrix (grid_ref_collint.h:697-762 as a matrix-vector product).  One task
+// per wave; every read of the cxyz precedes the wave barrier, every write
+// follows it.
+__global__ __launch_bounds__(256) void general_cijk_kernel(const Task *__restrict__ tasks,
+                                                           PointParams P) {
+  __shared__ double cx[4][84];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long t = (long long)blockIdx.x * 4 + wave;
+  if (t >= P.ntasks) return;
+  const long long task = launch_task(P, t);
+  const Task &T = tasks[task];
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  if (lp > 6) return;  // the per-task kernels take these
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  double *io = P.cxyz_all + task * P.cxyz_stride;
+  for (int e = lane; e < nc; e += 64) cx[wave][e] = io[e];
+  wave_sync();
+  const double *MT = P.gmat + P.gmat_off[T.level * 8 + lp] + nc * nc;
+  for (int o = lane; o < nc; o += 64) {
+    double s = 0.0;
+    for (int i = 0; i < nc; i++) s = fma(MT[i * nc + o], cx[wave][i], s);
+    io[o] = s;
+  }
+}
+
 // Collocation of general tasks, one task per wave, into the HBM level grids.
 template <int LPB>
 __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__restrict__ tasks,
@@ -373,7 +389,7 @@ __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__re
   const int NP = P.wp.np;
   const GenLds W = gen_lds(lds, P.wp, wave);
   const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
-  gen_cijk(P, T, task, lp, W, NP, lane);
+  gen_cijk(P, task, lp, W, lane);
   const GenStage S = gen_stage(W, NP);
   switch (lp) {
     case 0: gen_collocate_rows<0>(T, lay, W.cijk, NP, S, L.grid, lane); break;
@@ -413,7 +429,7 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
     const TileItem I = items[it];
     const Task &T = tasks[I.task];
     const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
-    gen_cijk(P, T, I.task, lp, W, NP, lane);
+    gen_cijk(P, I.task, lp, W, lane);
     switch (lp) {
       case 0: gen_tile_rows<0>(T, lay, W.cijk, NP, I, U, lds, lane); break;
       case 1: gen_tile_rows<1>(T, lay, W.cijk, NP, I, U, lds, lane); break;
@@ -490,6 +506,12 @@ static hipError_t launch_gen(K kernel, const PointParams &P, hipStream_t s) {
 
 // General tasks [task_base, task_base + ntasks) with lp <= lpb (4 or 6);
 // P.wp: np = lp bound + 1, two dense np^3 arrays per wave.
+hipError_t launch_general_cijk(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(general_cijk_kernel, dim3((P.ntasks + 3) / 4), dim3(256), 0, s, P.tasks, P);
+  return hipGetLastError();
+}
+
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.np > lpb + 1) return hipErrorInvalidValue;
