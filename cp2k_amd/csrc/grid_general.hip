@@ -29,6 +29,7 @@ namespace gm {
 namespace {
 
 constexpr int kRestart = 32;  // points between exact exp() evaluations of the recurrence
+constexpr int kBatch = 8;     // potential values loaded ahead per lane (integrate)
 
 struct GenLds {
   double *cxyz;  // dense [np][np][np]
@@ -289,16 +290,24 @@ __device__ __forceinline__ void gen_integrate_rows(const Task &T, const Layout &
         gen_gauss(T, lay, s0, j, k, g, q);
         const int n = min(b - s0 + 1, kRestart);
         double di = s0 - T.gp[0];
-        for (int s = 0; s < n; s++) {
-          double t = (double)src[s] * g;
+        // kBatch loads in flight per lane, unconditional (the index is
+        // clamped into the run); the points past the run count as 0
+        for (int s = 0; s < n; s += kBatch) {
+          R raw[kBatch];
 #pragma unroll
-          for (int il = 0; il <= LP; il++) {
-            ci[il] += t;
-            t *= di;
+          for (int u = 0; u < kBatch; u++) raw[u] = src[min(s + u, n - 1)];
+#pragma unroll
+          for (int u = 0; u < kBatch; u++) {
+            double t = (s + u < n ? (double)raw[u] : 0.0) * g;
+#pragma unroll
+            for (int il = 0; il <= LP; il++) {
+              ci[il] += t;
+              t *= di;
+            }
+            g *= q;
+            q *= r2step;
+            di += 1.0;
           }
-          g *= q;
-          q *= r2step;
-          di += 1.0;
         }
         src += n;
       }
