@@ -368,8 +368,17 @@ __device__ __forceinline__ void integrate_group(const PointParams &P, const Task
     R g[NCL];
 #pragma unroll
     for (int u = 0; u < NCL; u++) {
-      g[u] = 0;
-      if (gz < nz && (unsigned)(k - C[u].kl) < span[u]) g[u] = grid[gz * nxy + C[u].base];
+      const bool in = gz < nz && (unsigned)(k - C[u].kl) < span[u];
+      if constexpr (sizeof(R) == 4) {
+        // FP32: unconditional loads (index 0 stands in for a point outside
+        // the column) keep the unrolled iterations' loads in flight; FP64
+        // measured faster with the loads under the branch
+        const R v = grid[in ? gz * nxy + C[u].base : 0];
+        g[u] = in ? v : R(0);
+      } else {
+        g[u] = 0;
+        if (in) g[u] = grid[gz * nxy + C[u].base];
+      }
     }
 #pragma unroll
     for (int u = 0; u < NCL; u++)
