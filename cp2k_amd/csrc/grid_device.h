@@ -494,10 +494,12 @@ static __device__ __forceinline__ int pm(int a, int m) {
 
 static __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// a mod n for a within a few periods of [0, n).
+// a mod n (in [0, n)).  One period either side of [0, n) takes two selects;
+// anything farther (cells smaller than the task) the remainder.
 static __device__ __forceinline__ int wrap(int a, int n) {
-  while (a < 0) a += n;
-  while (a >= n) a -= n;
+  a = a < 0 ? a + n : a;
+  a = a >= n ? a - n : a;
+  if ((unsigned)a >= (unsigned)n) a = pm(a, n);
   return a;
 }
 
