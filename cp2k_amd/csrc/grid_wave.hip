@@ -294,6 +294,24 @@ __device__ __forceinline__ void tile_group(const PointParams &P, const Task &T, 
   const int kmin = __builtin_amdgcn_readfirstlane(wave_min(klmin));
   const int ka = max(kmin, (int)I.k0), kb = min(1 - kmin, (int)I.k1);
   int gz = wrap(cw[2] + ka, ngz);
+  const int tza = gz - U.gz0, nk = kb - ka + 1;
+  if (tza >= 0 && tza + nk <= U.bz && gz + nk <= ngz) {
+    // every plane inside the tile and no periodic wrap on the way: no test
+    // or wrap per plane
+    const R *pz = tabz + (ka - T.lo[2]) * NP;
+    double *wz = win + tza * nxby;
+#pragma unroll 2
+    for (int k = ka; k <= kb; k++, pz += NP, wz += nxby) {
+#pragma unroll
+      for (int u = 0; u < NCL; u++) {
+        R v = 0;
+#pragma unroll
+        for (int lz = 0; lz <= LP; lz++) v = fmar(D[u][lz], pz[lz], v);
+        if ((unsigned)(k - C[u].kl) < span[u]) atomicAdd(&wz[lbase[u]], (double)v);
+      }
+    }
+    return;
+  }
 #pragma unroll 2
   for (int k = ka; k <= kb; k++) {
     const R *pz = tabz + (k - T.lo[2]) * NP;
