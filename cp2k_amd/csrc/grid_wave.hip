@@ -380,29 +380,51 @@ __device__ __forceinline__ void integrate_group(const PointParams &P, const Task
     for (int l = 0; l <= LP; l++) M[u][l] = 0;
   }
   int gz = wrap(cw[2] + kmin, ngz);
+  const int nk = 2 - 2 * kmin;
+  if (sizeof(R) == 4 && gz + nk <= nz && gz + nk <= ngz) {
+    // FP32, every plane inside the local grid and no periodic wrap on the
+    // way: no test or wrap per plane (FP64 measured slower with this copy)
+    const R *pz = tabz + (kmin - T.lo[2]) * NP;
+    const R *gp = grid + gz * nxy;
 #pragma unroll 4  // several grid loads in flight per wave
-  for (int k = kmin; k <= 1 - kmin; k++) {
-    const R *pz = tabz + (k - T.lo[2]) * NP;
-    R g[NCL];
+    for (int k = kmin; k <= 1 - kmin; k++, pz += NP, gp += nxy) {
+      R g[NCL];
 #pragma unroll
-    for (int u = 0; u < NCL; u++) {
-      const bool in = gz < nz && (unsigned)(k - C[u].kl) < span[u];
-      if constexpr (sizeof(R) == 4) {
-        // FP32: unconditional loads (index 0 stands in for a point outside
-        // the column) keep the unrolled iterations' loads in flight; FP64
-        // measured faster with the loads under the branch
-        const R v = grid[in ? gz * nxy + C[u].base : 0];
+      for (int u = 0; u < NCL; u++) {
+        const bool in = (unsigned)(k - C[u].kl) < span[u];
+        const R v = gp[in ? C[u].base : 0];
         g[u] = in ? v : R(0);
-      } else {
-        g[u] = 0;
-        if (in) g[u] = grid[gz * nxy + C[u].base];
       }
+#pragma unroll
+      for (int u = 0; u < NCL; u++)
+#pragma unroll
+        for (int l = 0; l <= LP; l++) M[u][l] = fmar(g[u], pz[l], M[u][l]);
     }
+  } else {
+#pragma unroll 4  // several grid loads in flight per wave
+    for (int k = kmin; k <= 1 - kmin; k++) {
+      const R *pz = tabz + (k - T.lo[2]) * NP;
+      R g[NCL];
 #pragma unroll
-    for (int u = 0; u < NCL; u++)
+      for (int u = 0; u < NCL; u++) {
+        const bool in = gz < nz && (unsigned)(k - C[u].kl) < span[u];
+        if constexpr (sizeof(R) == 4) {
+          // FP32: unconditional loads (index 0 stands in for a point outside
+          // the column) keep the unrolled iterations' loads in flight; FP64
+          // measured faster with the loads under the branch
+          const R v = grid[in ? gz * nxy + C[u].base : 0];
+          g[u] = in ? v : R(0);
+        } else {
+          g[u] = 0;
+          if (in) g[u] = grid[gz * nxy + C[u].base];
+        }
+      }
 #pragma unroll
-      for (int l = 0; l <= LP; l++) M[u][l] = fmar(g[u], pz[l], M[u][l]);
-    gz = (gz + 1 == ngz) ? 0 : gz + 1;
+      for (int u = 0; u < NCL; u++)
+#pragma unroll
+        for (int l = 0; l <= LP; l++) M[u][l] = fmar(g[u], pz[l], M[u][l]);
+      gz = (gz + 1 == ngz) ? 0 : gz + 1;
+    }
   }
 #pragma unroll
   for (int u = 0; u < NCL; u++) {
