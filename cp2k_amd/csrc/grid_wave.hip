@@ -150,10 +150,23 @@ __device__ void wave_cxyz(const PointParams &P, const Task &T, const Group &G, i
     const double *ax = &alpha[((0 * nB + B.x) * nA + A.x) * np];
     const double *ay = &alpha[((1 * nB + B.y) * nA + A.y) * np];
     const double *az = &alpha[((2 * nB + B.z) * nA + A.z) * np];
-    for (int lz = 0; lz <= A.z + B.z; lz++)
-      for (int ly = 0; ly <= A.y + B.y; ly++)
-        for (int lx = 0; lx <= A.x + B.x; lx++)
-          atomicAdd(&W.cxyz[(lz * NP + ly) * NP + lx], cv * (ax[lx] * ay[ly] * az[lz] * pref));
+    // the (lz, ly, lx) terms of the pair, started at a lane-dependent term
+    // and wrapped around: lanes that all began at (0, 0, 0) would add to the
+    // same LDS address in the same instruction
+    const int nxl = A.x + B.x + 1, nyl = A.y + B.y + 1, nzl = A.z + B.z + 1;
+    const int cnt = nxl * nyl * nzl;
+    int m0 = lane % cnt;
+    int lx = m0 % nxl, ly = (m0 / nxl) % nyl, lz = m0 / (nxl * nyl);
+    for (int m = 0; m < cnt; m++) {
+      atomicAdd(&W.cxyz[(lz * NP + ly) * NP + lx], cv * (ax[lx] * ay[ly] * az[lz] * pref));
+      if (++lx == nxl) {
+        lx = 0;
+        if (++ly == nyl) {
+          ly = 0;
+          if (++lz == nzl) lz = 0;
+        }
+      }
+    }
   }
   wave_sync();
 }
