@@ -405,6 +405,33 @@ __device__ __forceinline__ double vb_plain(Ang a, Ang b, int i, int j, double zb
     }                                                                 \
   } while (0)
 
+// Forces (and virial) of one (a, b) pair without tau: the plain getters
+// above with each cab element looked up once (c(a, b), c(a + 1_i, b) and
+// c(a + 1_i + 1_j, b) serve several of them).
+__device__ __forceinline__ void pair_forces_plain(Ang a, Ang b, double za, double zb,
+                                                  const double rab[3], const CabGet &c,
+                                                  double pv, bool virial, double *f, double *v) {
+  const double *row = c.m + aidx(b) * c.n1;  // c(x, b) = row[aidx(x)]
+  const double c0 = row[aidx(a)];
+  double cu[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    cu[i] = row[aidx(up(a, i))];
+    f[i] += pv * (2.0 * za * cu[i] - comp(a, i) * row[aidx(dn(a, i))]);
+    f[3 + i] += pv * (2.0 * zb * (cu[i] - rab[i] * c0) - comp(b, i) * c(a, dn(b, i)));
+  }
+  if (!virial) return;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double cuu = row[aidx(up(up(a, j), i))];
+      v[i * 3 + j] += pv * (2.0 * za * cuu - comp(a, j) * row[aidx(up(dn(a, j), i))]);
+      v[i * 3 + j] += pv * (2.0 * zb * (cuu - cu[i] * rab[j] - cu[j] * rab[i] + c0 * rab[j] * rab[i]) -
+                            comp(b, j) * c(a, up(dn(b, j), i)));
+    }
+}
+
 static __device__ double get_hab(Ang a, Ang b, double za, double zb, const CabGet &c, bool tau) {
   if (!tau) return c(a, b);
   double s = 0.0;

@@ -557,7 +557,10 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
     const Ang A = orb(ia), B = orb(jb);
     const long long q = G.cart_off + (long long)(T.o2 + jb) * G.ncoa + T.o1 + ia;
     if (q < P.cart_len) atomicAdd(&P.hab_cart[q], get_hab(A, B, za, zb, cg, tau));
-    if (FULL && P.do_forces) {
+    if (FULL && P.do_forces && !tau) {
+      const double pv = (q < P.cart_len) ? P.cart[q] : 0.0;
+      pair_forces_plain(A, B, za, zb, T.rab, cg, pv, P.do_virial != 0, f, v);
+    } else if (FULL && P.do_forces) {
       const double pv = (q < P.cart_len) ? P.cart[q] : 0.0;
       for (int i = 0; i < 3; i++) {
         f[i] += pv * get_fa(A, B, i, za, zb, cg, tau);
