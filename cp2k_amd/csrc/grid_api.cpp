@@ -2024,21 +2024,38 @@ struct BatchImpl {
   std::vector<int> level_off, atom_off;  // per list (+ total)
   std::vector<long long> pab_off;        // per list (+ total), doubles
   std::vector<long long> grid_off;       // per merged level (+ total), doubles
-  DevBuf pab, hab, rho, pot, forces, virial, vptr, vent, lev_col, lev_int;
+  DevBuf pab, hab, rho, pot, forces, virial, vptr, vent;
   std::vector<double *> rho_lv, pot_lv;  // per merged level
+  // A captured run with its own level tables (the FP32 potential pointer in
+  // the integrate table depends on the run's precision).
   struct Graph {
     hipGraphExec_t exec = nullptr;
+    DevBuf lev_col, lev_int;
   };
   std::map<std::array<int, 5>, Graph> graphs;  // (what, func, tau, forces, precision)
-  ~BatchImpl() {
-    for (auto &kv : graphs)
+  void drop_graphs() {
+    for (auto &kv : graphs) {
       if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-    DevBuf *bufs[] = {&pab, &hab, &rho, &pot, &forces, &virial, &vptr, &vent, &lev_col, &lev_int};
+      kv.second.lev_col.release();
+      kv.second.lev_int.release();
+    }
+    graphs.clear();
+  }
+  ~BatchImpl() {
+    drop_graphs();
+    DevBuf *bufs[] = {&pab, &hab, &rho, &pot, &forces, &virial, &vptr, &vent};
     for (DevBuf *b : bufs) b->release();
     delete merged;
     magic = 0;
   }
 };
+
+// Device buffers of the merged list that a captured graph addresses and that
+// a call may (re)allocate on demand; a graph is valid while none has moved.
+std::array<const void *, 6> captured_buffers(const TaskListImpl *tl) {
+  return {tl->d_cxyz.p, tl->d_gridf.p, tl->d_cart.p, tl->d_hab_cart.p, tl->d_task_out.p,
+          tl->d_gitems.p};
+}
 
 BatchImpl *checked_batch(const void *p) {
   BatchImpl *b = (BatchImpl *)p;
@@ -2064,16 +2081,16 @@ void static_levels(BatchImpl *b, DevBuf &dst, const std::vector<double *> &grids
 
 // The merged calls of a run: per batch, collocate (what & 1) then integrate
 // (what & 2).
-void batch_enqueue(BatchImpl *b, int what, int func, bool tau, bool forces, bool fixed,
-                   hipStream_t s) {
+void batch_enqueue(BatchImpl *b, int what, int func, bool tau, bool forces,
+                   const BatchImpl::Graph *fixed, hipStream_t s) {
   TaskListImpl *tl = b->merged;
   double *pab = (double *)b->pab.p;
   if (what & 1) {
-    tl->static_levels = fixed ? (const Level *)b->lev_col.p : nullptr;
+    tl->static_levels = fixed ? (const Level *)fixed->lev_col.p : nullptr;
     run_collocate(tl, func, pab, b->rho_lv.data(), s);
   }
   if (what & 2) {
-    tl->static_levels = fixed ? (const Level *)b->lev_int.p : nullptr;
+    tl->static_levels = fixed ? (const Level *)fixed->lev_int.p : nullptr;
     const ListVirial lv{(const int *)b->vptr.p, (const int *)b->vent.p, b->nlists};
     run_integrate(tl, tau, forces ? pab : nullptr, b->pot_lv.data(), (double *)b->hab.p,
                   b->pab_off[b->nlists], forces ? (double *)b->forces.p : nullptr,
@@ -2714,24 +2731,29 @@ void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what, const enum 
   auto it = b->graphs.find(key);
   if (it == b->graphs.end()) {
     // an eager run allocates what the calls allocate on first use and
-    // builds the cached launch plans; the capture then holds kernels only
-    batch_enqueue(b, what, func, compute_tau, forces, false, s);
+    // builds the cached launch plans; the capture then holds kernels only.
+    // If that run moved a buffer (e.g. the per-task cxyz buffer grows with
+    // the l shift of the call), the graphs captured before address freed
+    // memory: they are all dropped and captured again on their next use.
+    const auto before = captured_buffers(b->merged);
+    batch_enqueue(b, what, func, compute_tau, forces, nullptr, s);
     sync_stream(s);
-    static_levels(b, b->lev_col, b->rho_lv, nullptr);
-    static_levels(b, b->lev_int, b->pot_lv,
+    if (captured_buffers(b->merged) != before) b->drop_graphs();
+    BatchImpl::Graph &g = b->graphs[key];
+    static_levels(b, g.lev_col, b->rho_lv, nullptr);
+    static_levels(b, g.lev_int, b->pot_lv,
                   precision() == GRID_MI355X_FP32_MIXED ? (const float *)b->merged->d_gridf.p
                                                         : nullptr);
     hipGraph_t graph = nullptr;
     S.capturing = true;
     HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    batch_enqueue(b, what, func, compute_tau, forces, true, s);
+    batch_enqueue(b, what, func, compute_tau, forces, &g, s);
     const hipError_t e = hipStreamEndCapture(s, &graph);
     S.capturing = false;
     HIP_CHECK(e);
-    BatchImpl::Graph g;
     HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
     HIP_CHECK(hipGraphDestroy(graph));
-    it = b->graphs.emplace(key, g).first;
+    it = b->graphs.find(key);
   }
   TimedLaunch tm(14, s);
   HIP_CHECK(hipGraphLaunch(it->second.exec, s));

@@ -113,3 +113,71 @@ def test_batch_fp32_mixed(lists):
                 assert r["forces_linf"] <= precision.FORCE_BAR, (rep, i, r)
     finally:
         grid.set_precision(grid.FP64)
+
+
+def _small(lists):
+    # two lists are enough: the cubic one-level list and the water box
+    return [lists[0], lists[4]]
+
+
+def test_batch_graphs_survive_buffer_growth(lists):
+    """Runs whose l shift grows the per-task buffers (collocate, integrate
+    with forces and virial, tau) alternate; every replay of an earlier graph
+    must still match the eager calls."""
+    sub = _small(lists)
+    gts = [grid.GridTaskList(a) for a, _, _ in sub]
+    batch = grid.GridBatch(gts)
+    for i, (a, p, V) in enumerate(sub):
+        batch.set_inputs(i, pab=p, V=V)
+    want_rho = [g.collocate(100, p) for g, (_, p, _) in zip(gts, sub)]
+    want_f = [g.integrate(V, p, forces=True, virial=True) for g, (_, p, V) in zip(gts, sub)]
+    want_tau = [g.integrate(V, p, compute_tau=True) for g, (_, p, V) in zip(gts, sub)]
+    seq = [(1, False, False), (2, False, True), (1, False, False), (3, True, False),
+           (2, False, True), (1, False, False)]
+    for step, (what, tau, forces) in enumerate(seq):
+        print("batch sequence", step, what, tau, forces, flush=True)
+        batch.run(what, compute_tau=tau, forces=forces)
+        grid.synchronize()
+        for i in range(len(sub)):
+            rho, hab, f, v = batch.outputs(i)
+            if what & 1:
+                for x, y in zip(rho, want_rho[i]):
+                    assert replay.rel_err(x, y) < TOL, (step, i)
+            if what & 2:
+                w = want_tau[i] if tau else want_f[i]
+                assert replay.rel_err(hab, w[0]) < TOL, (step, i)
+                if forces:
+                    assert replay.rel_err(f, w[1]) < FTOL, (step, i)
+                    assert replay.rel_err(v, w[2]) < FTOL, (step, i)
+
+
+def test_batch_graphs_alternate_precision(lists):
+    """FP32-mixed and FP64 integrate graphs of one batch hold their own level
+    tables: replays alternate precision and each matches the eager call of
+    its precision."""
+    sub = _small(lists)
+    gts = [grid.GridTaskList(a) for a, _, _ in sub]
+    batch = grid.GridBatch(gts)
+    for i, (a, p, V) in enumerate(sub):
+        batch.set_inputs(i, pab=p, V=V)
+    want = {}
+    try:
+        for prec in (grid.FP64, grid.FP32_MIXED):
+            grid.set_precision(prec)
+            want[prec] = [g.integrate(V, p, forces=True, virial=True)
+                          for g, (_, p, V) in zip(gts, sub)]
+        for step, prec in enumerate([grid.FP32_MIXED, grid.FP64, grid.FP32_MIXED, grid.FP64,
+                                     grid.FP32_MIXED]):
+            print("batch precision", step, prec, flush=True)
+            grid.set_precision(prec)
+            batch.run(2, forces=True)
+            grid.synchronize()
+            for i in range(len(sub)):
+                _, hab, f, v = batch.outputs(i)
+                w = want[prec][i]
+                # eager and replay run the same kernels: FP64 summation order only
+                assert replay.rel_err(hab, w[0]) < TOL, (step, i)
+                assert replay.rel_err(f, w[1]) < FTOL, (step, i)
+                assert replay.rel_err(v, w[2]) < FTOL, (step, i)
+    finally:
+        grid.set_precision(grid.FP64)
