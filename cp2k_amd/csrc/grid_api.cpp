@@ -54,8 +54,10 @@ hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s);
 hipError_t launch_finish_block(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_cijk(const PointParams &P, hipStream_t s);
-hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s);
+hipError_t launch_general_integrate(const PointParams &P, int lo, int hi, hipStream_t s);
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
+double *noalloc_scratch(long long *doubles, hipError_t *err);
+hipError_t launch_write_levels(const Level *h, int n, Level *dst, hipStream_t s);
 }  // namespace gm
 
 using namespace gm;
@@ -107,6 +109,7 @@ struct LibState {
   bool stream_set = false;
   int precision = -1;
   bool capturing = false;  // a batch graph is being captured: no timing events
+  bool noalloc = false;    // inside the allocation-free entry point: no timing events
   int wave_kernels = -1;  // wave-per-task kernels for orthorhombic tasks
   bool timing = false;
   double kernel_ms[kNumKernelIds] = {};
@@ -114,8 +117,20 @@ struct LibState {
   std::vector<PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
   long counters[4][kCounterLp] = {};
+  long launches[kNumKernelIds][2] = {};  // eager point-kernel launches: [id][FP64, FP32]
 } S;
 
+}  // namespace
+
+namespace gm {
+// Called by the launchers of the point kernels: one launch of kernel id
+// (the timing ids) with FP32 (true) or FP64 point evaluation.
+void count_launch(int id, bool fp32) {
+  if (id >= 0 && id < kNumKernelIds) S.launches[id][fp32 ? 1 : 0]++;
+}
+}  // namespace gm
+
+namespace {
 hipStream_t stream() {
   if (S.stream_set) return S.user_stream;
   if (!S.own_stream) HIP_CHECK(hipStreamCreateWithFlags(&S.own_stream, hipStreamNonBlocking));
@@ -157,7 +172,7 @@ struct TimedLaunch {
   hipEvent_t a = nullptr, b = nullptr;
   hipStream_t s;
   TimedLaunch(int id_, hipStream_t s_) : id(id_), s(s_) {
-    if (S.timing && !S.capturing && id >= 0) {
+    if (S.timing && !S.capturing && !S.noalloc && id >= 0) {
       a = get_event();
       b = get_event();
       HIP_CHECK(hipEventRecord(a, s));
@@ -322,6 +337,9 @@ struct OrthoLevel {
   // launch shape of the tile kernels (grid_mi355x_autotune): workgroup
   // rounds per CU over the level's items, largest waves per workgroup
   int rounds = 4, max_waves = 12;
+  // general tasks of the level: largest index box per axis and rows (j, k)
+  int gni = 0, gnj = 0, gnk = 0;
+  long long grows = 0;
 };
 
 // Items of one tile, sorted by lp0: start[v] = first item with lp0 >= v.
@@ -349,6 +367,8 @@ enum MergedList {
   kMSetupHi, kMOrthoC, kMDirectC,  // collocation of classed orthorhombic tasks with lp 7..8
   kMOrthoRestC,          // orthorhombic tasks the per-task kernels collocate
   kMOrthoHl,             // integration of classed orthorhombic tasks with lp 7..11
+  kMGenHl,               // integration of general tasks with lp 7..11
+  kMGenRestI,            // general tasks the per-task kernels integrate
   kMNum
 };
 struct MergedPlan {
@@ -473,6 +493,7 @@ void upload(DevBuf &b, const std::vector<T> &v) {
 }
 
 int lds_limit();
+TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false);
 
 void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevels,
                      int natoms, int nkinds, int nblocks, const int *block_offsets,
@@ -1024,6 +1045,13 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     OrthoLevel &g = tl->glev[T.level];
     if (g.end == 0) g.begin = s;
     g.end = s + 1;
+    const int ni = T.hi[0] - T.lo[0] + 1, nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
+    g.gni = std::max(g.gni, ni);
+    g.gnj = std::max(g.gnj, nj);
+    g.gnk = std::max(g.gnk, nk);
+    g.grows = std::max(g.grows, (long long)nj * nk);
+    if (ni > 65535 || nj > 4096)
+      fatal("task %d: index box %d x %d x %d is too large for the general kernels", s + 1, ni, nj, nk);
   }
   // --- collocation tiles of the general tasks: full rows of stride nx | 1;
   //     items are runs of rows j and planes k of a task's index box whose
@@ -1132,14 +1160,27 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   //     its transpose ---
   if (tl->has_general) {
     std::vector<double> gmat;
-    std::vector<int> goff((size_t)nlevels * 8, 0);
-    for (int l = 0; l < nlevels; l++)
-      for (int lp = 0; lp <= 6; lp++) {
-        goff[(size_t)l * 8 + lp] = (int)gmat.size();
+    std::vector<int> goff((size_t)nlevels * kGmatSlots, 0);
+    // lp up to the largest of the list plus the l shift of integrate with
+    // forces, virial and tau (5), at most 11
+    const int lp_top = std::min(kGmatSlots - 1, tl->max_lp + 5);
+    for (int l = 0; l < nlevels; l++) {
+      if (tl->glev[l].end <= tl->glev[l].begin) continue;
+      for (int lp = 0; lp <= lp_top; lp++) {
+        goff[(size_t)l * kGmatSlots + lp] = (int)gmat.size();
         append_general_matrix(tl->layouts[l].dh, lp, gmat);
       }
+    }
+    if (gmat.size() >= (1ULL << 31)) fatal("general cell matrices exceed 2^31 doubles");
     upload(tl->d_gmat, gmat);
     upload(tl->d_gmat_off, goff);
+  }
+
+  // --- collocation tile plans of every l shift a grid_func can ask for
+  //     (0..4), so that a collocate call allocates nothing for them ---
+  for (int sh = 0; sh <= 4; sh++) {
+    if (!tl->tiles.empty()) tile_plan(tl, sh);
+    if (!tl->gtiles.empty()) tile_plan(tl, sh, true);
   }
 
   // --- force CSR: atom -> (2*task + side) ---
@@ -1337,7 +1378,7 @@ int cxyz_stride(const TaskListImpl *tl, int sh) {
 // an lp bucket are dealt to a number of workgroups proportional to their
 // count, about four workgroups per CU over a level (one is resident per CU;
 // several rounds keep the last one short).
-TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false) {
+TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general) {
   auto &plans = general ? tl->gtile_plans : tl->tile_plans;
   const std::vector<TileInfo> &tiles = general ? tl->gtiles : tl->tiles;
   const std::vector<OrthoLevel> &lev = general ? tl->glev : tl->olev;
@@ -1483,6 +1524,52 @@ WavePlan general_plan(int lp) {
   return w;
 }
 
+// LDS plan of the general column walk (integrate) over the general tasks of
+// `level` (-1: all levels) with lp <= `lp`: the task's sums (and for lp > 6
+// the MFMA staging), the ratio tables, then the row table of a slab of
+// planes, the footprint and the column starts of the rows j.
+WavePlan general_col_plan(const TaskListImpl *tl, int level, int lp) {
+  int ni = 1, nj = 1, nk = 1;
+  long long rows = 1;
+  for (int l = 0; l < tl->nlevels; l++) {
+    if (level >= 0 && l != level) continue;
+    const OrthoLevel &g = tl->glev[l];
+    if (g.end <= g.begin) continue;
+    ni = std::max(ni, g.gni);
+    nj = std::max(nj, g.gnj);
+    nk = std::max(nk, g.gnk);
+    rows = std::max(rows, g.grows);
+  }
+  WavePlan w{};
+  w.level = level;
+  w.np = lp + 1;
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  w.o_cxyz = 0;
+  w.o_u = nc;
+  const int nd = nc + (lp > 6 ? 64 * 40 : 0);
+  w.o_q = 8 * nd;
+  w.q_cap = ni + nj + nk;
+  w.o_rows = w.o_q + 8 * w.q_cap;
+  // a slab holds at least one plane; 4096 rows (16 KiB) cover most tasks whole
+  w.rows_cap = (int)std::max<long long>(nj, std::min<long long>(rows, 4096));
+  w.o_jinfo = (w.o_rows + 4 * w.rows_cap + 15) & ~15;
+  w.j_cap = nj;
+  w.o_cstart = w.o_jinfo + 16 * nj;
+  w.wave_bytes = (w.o_cstart + 4 * (nj + 1) + 15) & ~15;
+  const int limit = lds_limit();
+  w.nwaves = 0;
+  for (int nw : {4, 2, 1})
+    if (nw * w.wave_bytes <= limit) {
+      w.nwaves = nw;
+      break;
+    }
+  if (!w.nwaves)
+    fatal("general integrate needs %d bytes of LDS per task, the device has %d", w.wave_bytes,
+          limit);
+  w.bytes = w.nwaves * w.wave_bytes;
+  return w;
+}
+
 // LDS plan of a general tile launch over `level` (-1: every tiled general
 // level): the largest tile (row stride nx | 1) and wave regions holding the
 // packed cxyz and the dense cijk.
@@ -1542,8 +1629,10 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
     const OrthoLevel &g = tl->glev[l];
     if (g.end <= g.begin) continue;
     auto first = [&](int v) { return g.lp_start[std::min(std::max(v, 0), kLp0Slots - 1)]; };
-    int c1 = first(5 - sh), c2 = first(7 - sh);  // lp <= 4 | lp 5..6 | larger
-    if (!wave_kernels()) c1 = c2 = g.begin;
+    // lp <= 4 | lp 5..6 | lp 7..11 (integrate) | larger
+    int c1 = first(5 - sh), c2 = first(7 - sh);
+    int c3 = mode == kIntegrate ? first(12 - sh) : c2;
+    if (!wave_kernels()) c1 = c2 = c3 = g.begin;
     if (c2 > g.begin) {
       const int lp_all = std::min(6, tl->lp0s[c2 - 1] + sh);
       PointParams Q = P;
@@ -1560,12 +1649,25 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
           HIP_CHECK(launch_cxyz_wave(Q, s));
         HIP_CHECK(launch_general_cijk(Q, s));  // the general kernels take cijk
       }
+      if (mode == kIntegrate) {
+        // one launch per lp: the lane sums take the registers that lp needs
+        for (int v = 0; v <= 6; v++) {
+          const int a = first(v - sh), b = v == 6 ? c2 : first(v + 1 - sh);
+          if (v < sh || b <= a) continue;
+          PointParams R = P;
+          R.task_base = a;
+          R.ntasks = b - a;
+          R.wp = general_col_plan(tl, l, v);
+          TimedLaunch tm(13, s);
+          HIP_CHECK(launch_general_integrate(R, v, v, s));
+        }
+      }
       const int cuts[2][2] = {{g.begin, c1}, {c1, c2}};
-      for (int r = 0; r < 2; r++) {
+      for (int r = 0; r < 2 && mode == kCollocate; r++) {
         const int a = cuts[r][0], b = cuts[r][1];
         if (b <= a) continue;
         const int lpb = r == 0 ? 4 : 6;
-        if (mode == kCollocate && g.tiled) {
+        if (g.tiled) {
           launch_general_tile_bucket(tl, P, l, r, tl->lp0s[b - 1] + sh, s);
           continue;
         }
@@ -1574,8 +1676,7 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         R.ntasks = b - a;
         R.wp = general_plan(std::min(lpb, tl->lp0s[b - 1] + sh));
         TimedLaunch tm(13, s);
-        HIP_CHECK(mode == kIntegrate ? launch_general_integrate(R, lpb, s)
-                                     : launch_general_collocate(R, lpb, s));
+        HIP_CHECK(launch_general_collocate(R, lpb, s));
       }
       if (mode == kIntegrate) {
         TimedLaunch tm(10, s);
@@ -1584,10 +1685,24 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         HIP_CHECK(launch_finish_wave(Q, s));
       }
     }
-    if (g.end > c2) {
+    if (c3 > c2) {  // integrate, lp 7..11: the column sums on the matrix cores
+      PointParams R = P;
+      R.task_base = c2;
+      R.ntasks = c3 - c2;
+      const int lp = std::min(11, tl->lp0s[c3 - 1] + sh);
+      R.wp = general_col_plan(tl, l, lp);
+      {
+        TimedLaunch tm(13, s);
+        HIP_CHECK(launch_general_integrate(R, 7, 11, s));
+      }
+      TimedLaunch tm(10, s);
+      R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanFinish);
+      HIP_CHECK(launch_finish_block(R, s));
+    }
+    if (g.end > c3) {
       PointParams Q = P;
-      Q.task_base = c2;
-      Q.ntasks = g.end - c2;
+      Q.task_base = c3;
+      Q.ntasks = g.end - c3;
       TimedLaunch tm(11, s);
       HIP_CHECK(mode == kIntegrate ? launch_integrate(Q, false, s) : launch_collocate(Q, false, s));
     }
@@ -1612,6 +1727,7 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
         if (!tl->glev[tl->task_level[t]].tiled) lists[lp <= 4 ? kMGenDirA : kMGenDirB].push_back(t);
       } else {
         lists[kMGenRest].push_back(t);
+        lists[(wave && lp <= 11) ? kMGenHl : kMGenRestI].push_back(t);
       }
     } else if (c == kClsOrthoClassed && wave && lp <= 6) {
       lists[kMSetup].push_back(t);
@@ -1736,10 +1852,25 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       PointParams R = sub(m);
       if (R.ntasks <= 0) continue;
       const int lpb = (m == kMGenA || m == kMGenDirA) ? 4 : 6;
-      R.wp = general_plan(std::min(lpb, mp.lp[m]));
+      const int lp = std::min(lpb, mp.lp[m]);
+      R.wp = mode == kIntegrate ? general_col_plan(tl, -1, lp) : general_plan(lp);
       TimedLaunch tm(13, s);
-      HIP_CHECK(mode == kIntegrate ? launch_general_integrate(R, lpb, s)
+      HIP_CHECK(mode == kIntegrate ? launch_general_integrate(R, lpb == 4 ? 0 : 5, lpb, s)
                                    : launch_general_collocate(R, lpb, s));
+    }
+    if (mode == kIntegrate) {
+      PointParams R = sub(kMGenHl);
+      if (R.ntasks > 0) {
+        const int lp = std::min(11, mp.lp[kMGenHl]);
+        R.wp = general_col_plan(tl, -1, lp);
+        {
+          TimedLaunch tm(13, s);
+          HIP_CHECK(launch_general_integrate(R, 7, 11, s));
+        }
+        TimedLaunch tm(10, s);
+        R.wp = make_wave_plan(tl, -1, lp, P.ld, kPlanFinish);
+        HIP_CHECK(launch_finish_block(R, s));
+      }
     }
     if (mode == kIntegrate) {
       PointParams H = sub(kMOrthoHl);
@@ -1758,11 +1889,12 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
       }
     }
   }
-  for (int m : {mode == kIntegrate ? kMOrthoRest : kMOrthoRestC, kMGenRest}) {
+  for (int m : {mode == kIntegrate ? kMOrthoRest : kMOrthoRestC,
+                mode == kIntegrate ? kMGenRestI : kMGenRest}) {
     PointParams R = sub(m);
     if (R.ntasks <= 0) continue;
     TimedLaunch tm(11, s);
-    const bool ortho = m != kMGenRest;
+    const bool ortho = m == kMOrthoRest || m == kMOrthoRestC;
     HIP_CHECK(mode == kIntegrate ? launch_integrate(R, ortho, s) : launch_collocate(R, ortho, s));
   }
 }
@@ -1796,8 +1928,16 @@ void launch_ranges(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   }
 }
 
+// Per-call device buffers of the allocation-free entry point, in its static
+// scratch array (nullptr members: the list's own buffers).
+struct CallScratch {
+  const Level *levels = nullptr;
+  double *cart = nullptr;
+  double *cxyz = nullptr;
+};
+
 // Geometry classes and the per-task cxyz buffer of the wave-per-task kernels.
-void set_wave_buffers(TaskListImpl *tl, PointParams &P) {
+void set_wave_buffers(TaskListImpl *tl, PointParams &P, const CallScratch *cs = nullptr) {
   P.geo = (const GeoClass *)tl->d_geo.p;
   P.geo_cols = (const int *)tl->d_geo_cols.p;
   P.geo_kmin = (const int *)tl->d_geo_kmin.p;
@@ -1806,8 +1946,10 @@ void set_wave_buffers(TaskListImpl *tl, PointParams &P) {
   P.cxyz_stride = cxyz_stride(tl, P.ld[0] + P.ld[2]);
   P.gmat = (const double *)tl->d_gmat.p;
   P.gmat_off = (const int *)tl->d_gmat_off.p;
-  P.cxyz_all = (double *)tl->d_cxyz.get(sizeof(double) * (size_t)P.cxyz_stride *
-                                        (size_t)std::max(tl->ntasks, 1));
+  P.cxyz_all = (cs && cs->cxyz) ? cs->cxyz
+                                 : (double *)tl->d_cxyz.get(sizeof(double) *
+                                                            (size_t)P.cxyz_stride *
+                                                            (size_t)std::max(tl->ntasks, 1));
 }
 
 long long level_points(const TaskListImpl *tl, int l) {
@@ -1866,42 +2008,44 @@ int decontract_chunk(const TaskListImpl *tl) {
   return std::max(1, std::min(std::max(tl->max_ncoa, 1), 4096 / nsb));
 }
 
-void run_decontract(TaskListImpl *tl, const double *pab, hipStream_t s) {
+void run_decontract(TaskListImpl *tl, const double *pab, hipStream_t s,
+                    double *cart = nullptr) {
   const int cach = decontract_chunk(tl);
   TimedLaunch tm(2, s);
   HIP_CHECK(launch_decontract((const Group *)tl->d_groups.p, tl->ngroups, pab, 0,
-                              (const double *)tl->d_sphi.p, (double *)tl->d_cart.p,
+                              (const double *)tl->d_sphi.p, cart ? cart : (double *)tl->d_cart.p,
                               tl->cart_len, cach, std::max(tl->max_nsb, 1), s));
 }
 
 void run_collocate(TaskListImpl *tl, int func, const double *pab, double *const *grids,
-                   hipStream_t s) {
+                   hipStream_t s, const CallScratch *cs = nullptr) {
   int ld[4];
   if (!func_ldiffs(func, ld)) fatal("unknown grid_func %d", func);
-  const Level *levels = set_levels(tl, grids, s);
+  const Level *levels = cs ? cs->levels : set_levels(tl, grids, s);
   {
     TimedLaunch tm(4, s);
     for (int l = 0; l < tl->nlevels; l++)
       HIP_CHECK(launch_fill(grids[l], level_points(tl, l), 0.0, s));
   }
-  run_decontract(tl, pab, s);
+  double *cart = cs ? cs->cart : (double *)tl->d_cart.p;
+  run_decontract(tl, pab, s, cart);
   PointParams P{};
   P.tasks = (const Task *)tl->d_tasks.p;
   P.ntasks = tl->ntasks;
   P.groups = (const Group *)tl->d_groups.p;
   P.levels = levels;
-  P.cart = (const double *)tl->d_cart.p;
+  P.cart = cart;
   P.cart_len = tl->cart_len;
   P.func = func;
   P.fp32 = precision() == GRID_MI355X_FP32_MIXED;
   for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
   P.plan = make_plan(tl, ld, false);
-  set_wave_buffers(tl, P);
+  set_wave_buffers(tl, P, cs);
   {
     TimedLaunch tm(0, s);
     launch_ranges(tl, P, s, kCollocate);
   }
-  end_levels(tl, s);
+  if (!cs) end_levels(tl, s);
   count_stats(tl, true, ld[0] + ld[2]);
 }
 
@@ -2459,6 +2603,63 @@ void grid_mi355x_collocate_task_list_dev(const grid_task_list *task_list,
   run_collocate(tl, func, pab_blocks, grids, s);
 }
 
+// No reference counterpart: grid_mi355x_collocate_task_list_dev without any
+// device allocation, for callers that cannot allocate (graph capture, tight
+// serving loops).  Per-call buffers come from the library's static scratch
+// array (128 MiB per device): the level table, the decontracted density
+// blocks and the per-task cxyz coefficients.  Lists that need more are
+// refused with a message; calls that overlap on one device race.
+void grid_mi355x_collocate_task_list_noalloc(const grid_task_list *task_list,
+                                             const enum grid_func func, const int nlevels,
+                                             const double *pab_blocks, double *const *grids) {
+  TaskListImpl *tl = checked(task_list);
+  if (nlevels != tl->nlevels) fatal("nlevels does not match the task list");
+  if (tl->empty) {
+    fatal("grid_mi355x_collocate_task_list_noalloc: the list is empty (zero the grids "
+          "with the caller's own means)");
+  }
+  if (tl->merged) fatal("grid_mi355x_collocate_task_list_noalloc: not for batched lists");
+  int ld[4];
+  if (!func_ldiffs(func, ld)) fatal("unknown grid_func %d", (int)func);
+  const int sh = ld[0] + ld[2];
+  if ((!tl->tiles.empty() && !tl->tile_plans.count(sh)) ||
+      (!tl->gtiles.empty() && !tl->gtile_plans.count(sh)))
+    fatal("grid_mi355x_collocate_task_list_noalloc: no tile plan for l shift %d", sh);
+  hipError_t err;
+  long long cap = 0;
+  double *base = noalloc_scratch(&cap, &err);
+  if (!base) fatal("scratch array address: %s", hipGetErrorString(err));
+  // layout: level table, decontracted blocks, cxyz (256-byte aligned parts)
+  auto up = [](long long n) { return (n + 31) & ~31LL; };
+  const long long n_lev = up(((long long)sizeof(Level) * tl->nlevels + 7) / 8);
+  const long long n_cart = up(std::max(tl->cart_len, 1LL));
+  const long long n_cxyz = up((long long)cxyz_stride(tl, sh) * std::max(tl->ntasks, 1));
+  if (n_lev + n_cart + n_cxyz > cap)
+    fatal("grid_mi355x_collocate_task_list_noalloc: the list needs %lld doubles of scratch "
+          "(%d tasks), the maximum is %lld (128 MiB); use "
+          "grid_mi355x_collocate_task_list_dev",
+          n_lev + n_cart + n_cxyz, tl->ntasks, cap);
+  CallScratch cs;
+  cs.levels = (const Level *)base;
+  cs.cart = base + n_lev;
+  cs.cxyz = base + n_lev + n_cart;
+  const hipStream_t s = stream();
+  constexpr int kMaxNoAllocLevels = 32;
+  if (tl->nlevels > kMaxNoAllocLevels)
+    fatal("grid_mi355x_collocate_task_list_noalloc: at most %d grid levels", kMaxNoAllocLevels);
+  Level lv[kMaxNoAllocLevels];
+  for (int l = 0; l < tl->nlevels; l++) {
+    lv[l].lay = tl->layouts[l];
+    lv[l].grid = grids[l];
+    lv[l].gridf = nullptr;
+    lv[l].npts = level_points(tl, l);
+  }
+  HIP_CHECK(launch_write_levels(lv, tl->nlevels, (Level *)base, s));
+  S.noalloc = true;
+  run_collocate(tl, func, pab_blocks, grids, s, &cs);
+  S.noalloc = false;
+}
+
 void grid_mi355x_integrate_task_list_dev(const grid_task_list *task_list, const bool compute_tau,
                                          const int natoms, const int nlevels,
                                          const double *pab_blocks, const double *const *grids,
@@ -2518,11 +2719,18 @@ void grid_mi355x_set_timing(const bool enable) {
   S.timing = enable;
 }
 
+void grid_mi355x_point_launches(const int kernel_id, long *fp64, long *fp32) {
+  if (kernel_id < 0 || kernel_id >= kNumKernelIds) fatal("kernel id %d out of range", kernel_id);
+  if (fp64) *fp64 = S.launches[kernel_id][0];
+  if (fp32) *fp32 = S.launches[kernel_id][1];
+}
+
 void grid_mi355x_reset_timing(void) {
   resolve_timings();
   for (int i = 0; i < kNumKernelIds; i++) {
     S.kernel_ms[i] = 0.0;
     S.kernel_n[i] = 0;
+    S.launches[i][0] = S.launches[i][1] = 0;
   }
 }
 
@@ -2815,6 +3023,10 @@ void grid_mi355x_autotune(const grid_task_list *task_list, const enum grid_func 
       o.max_waves = best_w;
       clear_plans();
     }
+  for (int sh = 0; sh <= 4; sh++) {  // as at create: every l shift planned
+    if (!tl->tiles.empty()) tile_plan(tl, sh);
+    if (!tl->gtiles.empty()) tile_plan(tl, sh, true);
+  }
   HIP_CHECK(hipEventDestroy(a));
   HIP_CHECK(hipEventDestroy(b));
 }

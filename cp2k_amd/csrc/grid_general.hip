@@ -26,10 +26,11 @@
 #include "grid_internal.h"
 
 namespace gm {
+void count_launch(int id, bool fp32);
+
 namespace {
 
 constexpr int kRestart = 32;  // points between exact exp() evaluations of the recurrence
-constexpr int kBatch = 8;     // potential values loaded ahead per lane (integrate)
 
 struct GenLds {
   double *cxyz;  // dense [np][np][np]
@@ -162,7 +163,7 @@ __device__ __forceinline__ GenStage gen_stage(const GenLds &W, int NP) {
   return g;
 }
 
-template <int LP>
+template <int LP, typename R>
 __device__ __forceinline__ void gen_collocate_rows(const Task &T, const Layout &lay,
                                                    const double *cijk, int NP,
                                                    const GenStage &S, double *grid, int lane) {
@@ -179,8 +180,11 @@ __device__ __forceinline__ void gen_collocate_rows(const Task &T, const Layout &
     long long base = 0;
     if (r < nj * nk && !gen_row(T, lay, j, k, i0, i1, base)) i1 = i0 - 1;
     if (r >= nj * nk) i1 = i0 - 1;
-    double ci[LP + 1];
-    gen_row_poly<LP>(cijk, j - T.gp[1], k - T.gp[2], ci);
+    double cd[LP + 1];
+    gen_row_poly<LP>(cijk, j - T.gp[1], k - T.gp[2], cd);
+    R ci[LP + 1];
+#pragma unroll
+    for (int il = 0; il <= LP; il++) ci[il] = (R)cd[il];
     const int len = i1 - i0 + 1;
     const int nch = -wave_min(-((len + kChunk - 1) / kChunk));  // chunks of the longest row
     int gx = wrap(i0 - lay.shift_local[0], ngx);
@@ -191,11 +195,12 @@ __device__ __forceinline__ void gen_collocate_rows(const Task &T, const Layout &
       if (c % (kRestart / kChunk) == 0 && ib <= i1) gen_gauss(T, lay, ib, j, k, g, q);
 #pragma unroll
       for (int s = 0; s < kChunk; s++) {
-        double p = ci[LP];
+        const R dr = (R)di;
+        R p = ci[LP];
 #pragma unroll
-        for (int il = LP - 1; il >= 0; il--) p = fma(p, di, ci[il]);
+        for (int il = LP - 1; il >= 0; il--) p = fmar(p, dr, ci[il]);
         const bool in = ib + s <= i1 && gx >= T.blo[0] && gx <= T.bhi[0];
-        S.val[lane * (kChunk + 1) + s] = p * g;
+        S.val[lane * (kChunk + 1) + s] = (double)(p * (R)g);
         S.idx[lane * (kChunk + 1) + s] = in ? rbase + gx : -1;
         g *= q;
         q *= r2step;
@@ -220,7 +225,7 @@ __device__ __forceinline__ int gen_tile_stride(int nx) { return nx | 1; }
 
 // Collocate the rows j0..j1, planes k0..k1 of a task (an item) into the LDS
 // tile of unit U: win[((gz - gz0) * by + gy - gy0) * sx + gx].
-template <int LP>
+template <int LP, typename R>
 __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, const double *cijk,
                                               int NP, const TileItem &I, const TileUnit &U,
                                               double *win, int lane) {
@@ -240,8 +245,11 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
     const int lz = wrap(k - lay.shift_local[2], lay.npts_global[2]) - U.gz0;
     if ((unsigned)ly >= (unsigned)U.by || (unsigned)lz >= (unsigned)U.bz) continue;  // not for host-built items
     double *row = win + (lz * U.by + ly) * sx;
-    double ci[LP + 1];
-    gen_row_poly<LP>(cijk, j - T.gp[1], k - T.gp[2], ci);
+    double cd[LP + 1];
+    gen_row_poly<LP>(cijk, j - T.gp[1], k - T.gp[2], cd);
+    R ci[LP + 1];
+#pragma unroll
+    for (int il = 0; il <= LP; il++) ci[il] = (R)cd[il];
     gen_runs(T, lay, i0, i1, [&](int a, int b, int gxa) {
       double *dst = row + gxa;
       for (int s0 = a; s0 <= b; s0 += kRestart) {
@@ -250,10 +258,11 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
         const int n = min(b - s0 + 1, kRestart);
         double di = s0 - T.gp[0];
         for (int s = 0; s < n; s++) {
-          double p = ci[LP];
+          const R dr = (R)di;
+          R p = ci[LP];
 #pragma unroll
-          for (int il = LP - 1; il >= 0; il--) p = fma(p, di, ci[il]);
-          atomicAdd(dst + s, p * g);
+          for (int il = LP - 1; il >= 0; il--) p = fmar(p, dr, ci[il]);
+          atomicAdd(dst + s, (double)(p * (R)g));
           g *= q;
           q *= r2step;
           di += 1.0;
@@ -264,91 +273,303 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
   }
 }
 
-// Integrate: lane-private acc[(kl, jl, il)] (packed, il fastest) of the
-// task's rows; R is the type the potential is read in.
+// ---------------------------------------------------------------------------
+// Integrate: column walk.  A wave takes one task.  Its lanes take
+// consecutive columns (i, j) of the task's footprint, x fastest, so that a
+// load instruction reads consecutive grid points of one or two rows; the
+// planes k are stepped together by the whole wave.  A point (i, j, k) belongs
+// to the task exactly when the reference's row (j, k) holds it
+// (grid_ref_collint.h:416-464 via gen_row: border mask, quadratic bounds);
+// the rows of a slab of planes are tabulated in LDS first:
+//   rows[k][j]   (i0 - lo0) | n << 16 for the points i0 .. i0 + n - 1
+//   jinfo[j]     footprint of row j over the slab {x0, width, k_lo, k_hi}
+//   cstart[j]    columns before row j (the footprint rows, j-major)
+// Along a column the Gaussian follows g(k + 1) = g(k) q(k), q(k + 1) =
+// q(k) exp(-2 zetp |h2|^2), with q(i, j, k) = qi[i] qj[j] qk[k] from three
+// tables; g is restarted from exp() every kColRestart planes.  Per point:
+// one potential value times g into the z moments M[kl] = sum V g dk^kl;
+// per column the moments times di^il dj^jl into the task's cijk sums
+// (lane-private registers for lp <= 6, the matrix cores for lp 7..11).
+// ---------------------------------------------------------------------------
+constexpr int kColRestart = 64;  // planes between exact exp() evaluations of g
+constexpr int kColStage = 40;    // MFMA staging per column: di^il[16], dj^jl[12], M[12]
+
+struct ColLds {
+  double *cijk;   // [NC] the task's reduced sums (packed, il fastest)
+  double *stage;  // [64 * kColStage] (lp > 6)
+  double *q;      // qi[ni], qj[nj], qk[nk]
+  int *rows;      // [planes of the slab][nj]
+  int4 *jinfo;    // [nj]
+  int *cstart;    // [nj + 1]
+};
+
+__device__ __forceinline__ ColLds col_lds(double *lds, const WavePlan &wp, int wave) {
+  char *b = reinterpret_cast<char *>(lds) + (size_t)wave * wp.wave_bytes;
+  ColLds w;
+  w.cijk = reinterpret_cast<double *>(b) + wp.o_cxyz;
+  w.stage = reinterpret_cast<double *>(b) + wp.o_u;
+  w.q = reinterpret_cast<double *>(b + wp.o_q);
+  w.rows = reinterpret_cast<int *>(b + wp.o_rows);
+  w.jinfo = reinterpret_cast<int4 *>(b + wp.o_jinfo);
+  w.cstart = reinterpret_cast<int *>(b + wp.o_cstart);
+  return w;
+}
+
+// One column's moments -> the task's sums, lane-private (lp <= 6):
+// acc[(kl, jl, il)] += M[kl] dj^jl di^il.
 template <int LP, typename R>
-__device__ __forceinline__ void gen_integrate_rows(const Task &T, const Layout &lay,
-                                                   const R *grid, double *acc, int lane) {
-  const int nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
-  double hh = 0.0;
+__device__ __forceinline__ void col_fold_regs(const R (&M)[LP + 1], double di, double dj,
+                                              double *acc) {
+  double pi[LP + 1], pj[LP + 1];
+  pi[0] = pj[0] = 1.0;
 #pragma unroll
-  for (int d = 0; d < 3; d++) hh += lay.dh[0][d] * lay.dh[0][d];
-  const double r2step = exp(-2.0 * T.zetp * hh);
-  RowWalk rw(lane, nj);
-  for (int r = lane; r < nj * nk; r += 64, rw.next()) {
-    const int j = T.lo[1] + rw.j, k = T.lo[2] + rw.k;
-    int i0, i1;
-    long long base;
-    if (!gen_row(T, lay, j, k, i0, i1, base)) continue;
-    double ci[LP + 1];
+  for (int l = 1; l <= LP; l++) {
+    pi[l] = pi[l - 1] * di;
+    pj[l] = pj[l - 1] * dj;
+  }
+  int e = 0;
 #pragma unroll
-    for (int il = 0; il <= LP; il++) ci[il] = 0.0;
-    gen_runs(T, lay, i0, i1, [&](int a, int b, int gxa) {
-      const R *src = grid + base + gxa;
-      for (int s0 = a; s0 <= b; s0 += kRestart) {
-        double g, q;
-        gen_gauss(T, lay, s0, j, k, g, q);
-        const int n = min(b - s0 + 1, kRestart);
-        double di = s0 - T.gp[0];
-        // kBatch loads in flight per lane, unconditional (the index is
-        // clamped into the run); the points past the run count as 0
-        for (int s = 0; s < n; s += kBatch) {
-          R raw[kBatch];
+  for (int kl = 0; kl <= LP; kl++)
 #pragma unroll
-          for (int u = 0; u < kBatch; u++) raw[u] = src[min(s + u, n - 1)];
+    for (int jl = 0; jl <= LP - kl; jl++) {
+      const double w = pj[jl] * (double)M[kl];
 #pragma unroll
-          for (int u = 0; u < kBatch; u++) {
-            double t = (s + u < n ? (double)raw[u] : 0.0) * g;
+      for (int il = 0; il <= LP - kl - jl; il++, e++) acc[e] = fma(pi[il], w, acc[e]);
+    }
+}
+
+typedef double col_double4 __attribute__((ext_vector_type(4)));
+
+// One group of 64 columns -> the task's sums on the matrix cores (lp 7..11):
+//   cijk[il][(jl, kl)] += sum_c di[c]^il (dj[c]^jl M[c][kl]),
+// v_mfma_f64_16x16x4f64 over 4 columns per step, M dimension il (padded to
+// 16), N dimension (jl, kl) in blocks of 16 (lz-major).  Operand layout:
+// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane
+// j + 16 (i % 4), register i / 4.
+template <int LP, typename R>
+__device__ __forceinline__ void col_fold_mfma(const R (&M)[LP + 1], double di, double dj,
+                                              double *stage, const int *qy, const int *qz,
+                                              col_double4 *acc, int lane) {
+  constexpr int NQ = (LP + 1) * (LP + 2) / 2;
+  constexpr int NB = (NQ + 15) / 16;
+  double *sc = stage + lane * kColStage;
+  double p = 1.0;
 #pragma unroll
-            for (int il = 0; il <= LP; il++) {
-              ci[il] += t;
-              t *= di;
+  for (int l = 0; l < 16; l++) {
+    sc[l] = l <= LP ? p : 0.0;
+    p *= di;
+  }
+  p = 1.0;
+#pragma unroll
+  for (int l = 0; l <= LP; l++) {
+    sc[16 + l] = p;
+    sc[28 + l] = (double)M[l];
+    p *= dj;
+  }
+  wave_sync();
+#pragma unroll 4
+  for (int kk = 0; kk < 16; kk++) {
+    const double *sk = stage + (4 * kk + (lane >> 4)) * kColStage;
+    const double a = sk[lane & 15];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+      const double b = qz[nb] <= LP ? sk[16 + qy[nb]] * sk[28 + qz[nb]] : 0.0;
+      acc[nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[nb], 0, 0, 0);
+    }
+  }
+  wave_sync();
+}
+
+// The task's points: slabs of planes, per slab the row table and the
+// footprint, then groups of 64 columns.  fold(M, di, dj) takes each
+// column's moments (every lane calls it, lanes without a column with M = 0).
+template <int LP, typename R, typename Fold>
+__device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const R *grid,
+                                         const WavePlan &wp, const ColLds &W, int lane,
+                                         Fold &&fold) {
+  const int ni = T.hi[0] - T.lo[0] + 1, nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
+  if (nj > wp.j_cap || nj > wp.rows_cap || ni + nj + nk > wp.q_cap) return;  // host-sized: never
+  const int ks = min(nk, wp.rows_cap / nj);  // planes per slab
+  const double *h0 = lay.dh[0], *h1 = lay.dh[1], *h2 = lay.dh[2];
+  const double g02 = h0[0] * h2[0] + h0[1] * h2[1] + h0[2] * h2[2];
+  const double g12 = h1[0] * h2[0] + h1[1] * h2[1] + h1[2] * h2[2];
+  const double g22 = h2[0] * h2[0] + h2[1] * h2[1] + h2[2] * h2[2];
+  // q(i, j, k) = exp(-zetp (2 r.h2 + h2.h2)), r.h2 = di g02 + dj g12 + dk g22
+  for (int e = lane; e < ni + nj + nk; e += 64) {
+    double arg;
+    if (e < ni)
+      arg = 2.0 * g02 * (T.lo[0] + e - T.gp[0]);
+    else if (e < ni + nj)
+      arg = 2.0 * g12 * (T.lo[1] + (e - ni) - T.gp[1]);
+    else
+      arg = 2.0 * g22 * (T.lo[2] + (e - ni - nj) - T.gp[2]) + g22;
+    W.q[e] = exp(-T.zetp * arg);
+  }
+  const double qq = exp(-2.0 * T.zetp * g22);
+  const int nx = lay.npts_local[0], nxy = nx * lay.npts_local[1];
+  const int ngx = lay.npts_global[0], ngy = lay.npts_global[1], ngz = lay.npts_global[2];
+  for (int kb = 0; kb < nk; kb += ks) {
+    const int kn = min(ks, nk - kb);
+    wave_sync();  // the previous slab's readers are done
+    for (int r = lane; r < nj * kn; r += 64) {
+      const int jj = r % nj, kk = r / nj;
+      int i0, i1;
+      long long base;
+      int v = 0;
+      if (gen_row(T, lay, T.lo[1] + jj, T.lo[2] + kb + kk, i0, i1, base))
+        v = (i0 - T.lo[0]) | ((i1 - i0 + 1) << 16);
+      W.rows[r] = v;
+    }
+    wave_sync();
+    for (int jj = lane; jj < nj; jj += 64) {
+      int a = 0x7fffffff, b = -1, klo = kn, khi = -1;
+      for (int kk = 0; kk < kn; kk++) {
+        const int v = W.rows[kk * nj + jj];
+        const int n = v >> 16;
+        if (n > 0) {
+          const int x0 = v & 0xffff;
+          a = min(a, x0);
+          b = max(b, x0 + n - 1);
+          klo = min(klo, kk);
+          khi = kk;
+        }
+      }
+      W.jinfo[jj] = b >= a ? make_int4(a, b - a + 1, klo, khi) : make_int4(0, 0, kn, -1);
+    }
+    wave_sync();
+    int ncol = 0;
+    for (int j0 = 0; j0 < nj; j0 += 64) {
+      const int jj = j0 + lane;
+      const int w = jj < nj ? W.jinfo[jj].y : 0;
+      int incl = w;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      if (jj < nj) W.cstart[jj] = ncol + incl - w;
+      ncol += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) W.cstart[nj] = ncol;
+    wave_sync();
+    int jj = 0;
+    for (int c0 = 0; c0 < ncol; c0 += 64) {
+      const int c = c0 + lane;
+      while (jj + 1 < nj && W.cstart[jj + 1] <= c) jj++;
+      const int4 J = W.jinfo[jj];
+      const int x = min(J.x + (c - W.cstart[jj]), ni - 1);
+      const int i = T.lo[0] + x, j = T.lo[1] + jj;
+      const int gx = wrap(i - lay.shift_local[0], ngx);
+      const int gy = wrap(j - lay.shift_local[1], ngy);
+      const bool act = c < ncol && gx >= T.blo[0] && gx <= T.bhi[0];
+      const int klo = wave_min(act ? J.z : kn);
+      const int khi = -wave_min(act ? -J.w : 1);
+      const double di = i - T.gp[0], dj = j - T.gp[1];
+      R M[LP + 1];
+#pragma unroll
+      for (int l = 0; l <= LP; l++) M[l] = R(0);
+      if (klo <= khi) {
+        const double qij = W.q[x] * W.q[ni + jj];
+        const int cb = gy * nx + gx;
+        for (int k0 = klo; k0 <= khi; k0 += kColRestart) {
+          const int k1 = min(khi, k0 + kColRestart - 1);
+          const int kabs = T.lo[2] + kb + k0;
+          double dk = kabs - T.gp[2];
+          double r2 = 0.0;
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const double rd = di * h0[d] + dj * h1[d] + dk * h2[d];
+            r2 += rd * rd;
+          }
+          double g = exp(-T.zetp * r2);
+          double q = qij * W.q[ni + nj + kb + k0];
+          int gz = wrap(kabs - lay.shift_local[2], ngz);
+          const int *rowk = W.rows + k0 * nj + jj;
+#pragma unroll 2
+          for (int kk = k0; kk <= k1; kk++, rowk += nj) {
+            const int v = *rowk;
+            const bool in = act && (unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16);
+            R val = R(0);
+            if (in) val = grid[gz * nxy + cb];
+            R t = (R)((double)val * g);
+            const R dkr = (R)dk;
+#pragma unroll
+            for (int l = 0; l <= LP; l++) {
+              M[l] += t;
+              t *= dkr;
             }
             g *= q;
-            q *= r2step;
-            di += 1.0;
+            q *= qq;
+            dk += 1.0;
+            gz = (gz + 1 == ngz) ? 0 : gz + 1;
           }
         }
-        src += n;
       }
-    });
-    const double dj = j - T.gp[1], dk = k - T.gp[2];
-    double dkp = 1.0;
-    int e = 0;
-#pragma unroll
-    for (int kl = 0; kl <= LP; kl++) {
-      double w = dkp;
-#pragma unroll
-      for (int jl = 0; jl <= LP - kl; jl++) {
-#pragma unroll
-        for (int il = 0; il <= LP - kl - jl; il++, e++) acc[e] = fma(ci[il], w, acc[e]);
-        w *= dj;
-      }
-      dkp *= dk;
+      fold(M, act ? di : 0.0, act ? dj : 0.0);
     }
   }
 }
 
-template <int LP, typename R>
-__device__ __forceinline__ void gen_integrate_task(const PointParams &P, const Task &T,
-                                                   const Layout &lay, const R *grid,
-                                                   const GenLds &W, int NP, long long task,
-                                                   int lane) {
-  constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
-  double acc[NC];
-#pragma unroll
-  for (int e = 0; e < NC; e++) acc[e] = 0.0;
-  gen_integrate_rows<LP, R>(T, lay, grid, acc, lane);
-  wave_sum_store<NC>(acc, W.cijk, lane);  // packed (il, jl, kl)
-  wave_sync();
-  // cxyz[i] = sum_o M[o][i] cijk[o], one output per lane
-  const double *M = P.gmat + P.gmat_off[T.level * 8 + LP];
+// cxyz = M^T cijk (the adjoint of the change of variables), one output per
+// lane, packed to the task's cxyz slot for the cxyz -> hab pass.
+__device__ __forceinline__ void col_cijk_to_cxyz(const PointParams &P, const Task &T, int lp,
+                                                 const double *cijk, long long task, int lane) {
+  const int nc = (lp + 1) * (lp + 2) * (lp + 3) / 6;
+  const double *M = P.gmat + P.gmat_off[T.level * kGmatSlots + lp];
   double *out = P.cxyz_all + task * P.cxyz_stride;
-  for (int i = lane; i < NC; i += 64) {
+  for (int i = lane; i < nc; i += 64) {
     double s = 0.0;
-    for (int o = 0; o < NC; o++) s = fma(M[o * NC + i], W.cijk[o], s);
+    for (int o = 0; o < nc; o++) s = fma(M[o * nc + i], cijk[o], s);
     out[i] = s;
   }
+}
+
+template <int LP, typename R>
+__device__ __forceinline__ void col_integrate_task(const PointParams &P, const Task &T,
+                                                   const Layout &lay, const R *grid,
+                                                   const ColLds &W, long long task, int lane) {
+  constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
+  if constexpr (LP <= 6) {
+    double acc[NC];
+#pragma unroll
+    for (int e = 0; e < NC; e++) acc[e] = 0.0;
+    col_walk<LP, R>(T, lay, grid, P.wp, W, lane,
+                    [&](const R(&M)[LP + 1], double di, double dj) {
+                      col_fold_regs<LP, R>(M, di, dj, acc);
+                    });
+    wave_sum_store<NC>(acc, W.cijk, lane);
+  } else {
+    constexpr int NQ = (LP + 1) * (LP + 2) / 2;
+    constexpr int NB = (NQ + 15) / 16;
+    int qy[NB], qz[NB];  // (jl, kl) of this lane's B column in each block
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+      int n = 16 * nb + (lane & 15), z = 0;
+      while (z <= LP && n >= LP - z + 1) {
+        n -= LP - z + 1;
+        z++;
+      }
+      qy[nb] = n;
+      qz[nb] = z;  // z > LP: padding column
+    }
+    col_double4 acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) acc[nb] = col_double4{0.0, 0.0, 0.0, 0.0};
+    col_walk<LP, R>(T, lay, grid, P.wp, W, lane,
+                    [&](const R(&M)[LP + 1], double di, double dj) {
+                      col_fold_mfma<LP, R>(M, di, dj, W.stage, qy, qz, acc, lane);
+                    });
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int il = 4 * r + (lane >> 4);
+        if (qz[nb] <= LP && il <= LP - qy[nb] - qz[nb])
+          W.cijk[packed_index(il, qy[nb], qz[nb], LP)] = acc[nb][r];
+      }
+  }
+  wave_sync();
+  col_cijk_to_cxyz(P, T, LP, W.cijk, task, lane);
 }
 
 }  // namespace
@@ -373,7 +594,7 @@ __global__ __launch_bounds__(256) void general_cijk_kernel(const Task *__restric
   double *io = P.cxyz_all + task * P.cxyz_stride;
   for (int e = lane; e < nc; e += 64) cx[wave][e] = io[e];
   wave_sync();
-  const double *MT = P.gmat + P.gmat_off[T.level * 8 + lp] + nc * nc;
+  const double *MT = P.gmat + P.gmat_off[T.level * kGmatSlots + lp] + nc * nc;
   for (int o = lane; o < nc; o += 64) {
     double s = 0.0;
     for (int i = 0; i < nc; i++) s = fma(MT[i * nc + o], cx[wave][i], s);
@@ -382,7 +603,7 @@ __global__ __launch_bounds__(256) void general_cijk_kernel(const Task *__restric
 }
 
 // Collocation of general tasks, one task per wave, into the HBM level grids.
-template <int LPB>
+template <int LPB, typename R>
 __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__restrict__ tasks,
                                                                 const Level *__restrict__ levels,
                                                                 PointParams P) {
@@ -401,13 +622,13 @@ __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__re
   gen_cijk(P, task, lp, W, lane);
   const GenStage S = gen_stage(W, NP);
   switch (lp) {
-    case 0: gen_collocate_rows<0>(T, lay, W.cijk, NP, S, L.grid, lane); break;
-    case 1: gen_collocate_rows<1>(T, lay, W.cijk, NP, S, L.grid, lane); break;
-    case 2: gen_collocate_rows<2>(T, lay, W.cijk, NP, S, L.grid, lane); break;
-    case 3: gen_collocate_rows<3>(T, lay, W.cijk, NP, S, L.grid, lane); break;
-    case 4: gen_collocate_rows<4>(T, lay, W.cijk, NP, S, L.grid, lane); break;
-    case 5: if (LPB >= 5) gen_collocate_rows<(LPB >= 5 ? 5 : 0)>(T, lay, W.cijk, NP, S, L.grid, lane); break;
-    case 6: if (LPB >= 6) gen_collocate_rows<(LPB >= 6 ? 6 : 0)>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 0: gen_collocate_rows<0, R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 1: gen_collocate_rows<1, R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 2: gen_collocate_rows<2, R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 3: gen_collocate_rows<3, R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 4: gen_collocate_rows<4, R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 5: if (LPB >= 5) gen_collocate_rows<(LPB >= 5 ? 5 : 0), R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
+    case 6: if (LPB >= 6) gen_collocate_rows<(LPB >= 6 ? 6 : 0), R>(T, lay, W.cijk, NP, S, L.grid, lane); break;
     default: break;  // the host routes larger lp to the per-task kernels
   }
 }
@@ -415,7 +636,7 @@ __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__re
 // Collocation of general tasks through LDS tiles: workgroup = unit
 // P.unit_base + blockIdx.x, the items U.ibeg + U.first + m * U.stride < U.iend
 // of one tile of level U.level; the tile is added to HBM once.
-template <int LPB>
+template <int LPB, typename R>
 __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
     const Task *__restrict__ tasks, const Level *__restrict__ levels,
     const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
@@ -440,13 +661,13 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
     const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
     gen_cijk(P, I.task, lp, W, lane);
     switch (lp) {
-      case 0: gen_tile_rows<0>(T, lay, W.cijk, NP, I, U, lds, lane); break;
-      case 1: gen_tile_rows<1>(T, lay, W.cijk, NP, I, U, lds, lane); break;
-      case 2: gen_tile_rows<2>(T, lay, W.cijk, NP, I, U, lds, lane); break;
-      case 3: gen_tile_rows<3>(T, lay, W.cijk, NP, I, U, lds, lane); break;
-      case 4: gen_tile_rows<4>(T, lay, W.cijk, NP, I, U, lds, lane); break;
-      case 5: if (LPB >= 5) gen_tile_rows<(LPB >= 5 ? 5 : 0)>(T, lay, W.cijk, NP, I, U, lds, lane); break;
-      case 6: if (LPB >= 6) gen_tile_rows<(LPB >= 6 ? 6 : 0)>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 0: gen_tile_rows<0, R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 1: gen_tile_rows<1, R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 2: gen_tile_rows<2, R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 3: gen_tile_rows<3, R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 4: gen_tile_rows<4, R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 5: if (LPB >= 5) gen_tile_rows<(LPB >= 5 ? 5 : 0), R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
+      case 6: if (LPB >= 6) gen_tile_rows<(LPB >= 6 ? 6 : 0), R>(T, lay, W.cijk, NP, I, U, lds, lane); break;
       default: break;  // the host routes larger lp to the per-task kernels
     }
     wave_sync();
@@ -465,9 +686,12 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
   }
 }
 
-// Integration of general tasks, one task per wave; writes the task's cxyz
-// (packed) for the cxyz -> hab pass.  R = float reads Level::gridf.
-template <int LPB, typename R>
+// Integration of general tasks, one task per wave (the column walk); writes
+// the task's cxyz (packed) for the cxyz -> hab pass.  R = float reads
+// Level::gridf and sums each column's moments in FP32.  The kernel covers
+// lp LO..HI: one exact lp keeps the register count (lane sums of lp <= 6)
+// at what that lp needs; LO >= 7 sums on the matrix cores.
+template <int LO, int HI, typename R>
 __global__ __launch_bounds__(256) void general_integrate_kernel(const Task *__restrict__ tasks,
                                                                 const Level *__restrict__ levels,
                                                                 PointParams P) {
@@ -480,24 +704,24 @@ __global__ __launch_bounds__(256) void general_integrate_kernel(const Task *__re
   const Task &T = tasks[task];
   const Level &L = levels[T.level];
   const Layout &lay = L.lay;
-  const int NP = P.wp.np;
-  const GenLds W = gen_lds(lds, P.wp, wave);
+  const ColLds W = col_lds(lds, P.wp, wave);
   const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
   const R *grid;
   if constexpr (sizeof(R) == 4)
     grid = L.gridf;
   else
     grid = L.grid;
+#define GEN_INT_CASE(V)                                                            \
+  case V:                                                                          \
+    if constexpr (V >= LO && V <= HI) col_integrate_task<V, R>(P, T, lay, grid, W, task, lane); \
+    break;
   switch (lp) {
-    case 0: gen_integrate_task<0, R>(P, T, lay, grid, W, NP, task, lane); break;
-    case 1: gen_integrate_task<1, R>(P, T, lay, grid, W, NP, task, lane); break;
-    case 2: gen_integrate_task<2, R>(P, T, lay, grid, W, NP, task, lane); break;
-    case 3: gen_integrate_task<3, R>(P, T, lay, grid, W, NP, task, lane); break;
-    case 4: gen_integrate_task<4, R>(P, T, lay, grid, W, NP, task, lane); break;
-    case 5: if (LPB >= 5) gen_integrate_task<(LPB >= 5 ? 5 : 0), R>(P, T, lay, grid, W, NP, task, lane); break;
-    case 6: if (LPB >= 6) gen_integrate_task<(LPB >= 6 ? 6 : 0), R>(P, T, lay, grid, W, NP, task, lane); break;
+    GEN_INT_CASE(0) GEN_INT_CASE(1) GEN_INT_CASE(2) GEN_INT_CASE(3) GEN_INT_CASE(4)
+    GEN_INT_CASE(5) GEN_INT_CASE(6) GEN_INT_CASE(7) GEN_INT_CASE(8) GEN_INT_CASE(9)
+    GEN_INT_CASE(10) GEN_INT_CASE(11)
     default: break;  // the host routes larger lp to the per-task kernels
   }
+#undef GEN_INT_CASE
 }
 
 template <typename K>
@@ -524,8 +748,13 @@ hipError_t launch_general_cijk(const PointParams &P, hipStream_t s) {
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.np > lpb + 1) return hipErrorInvalidValue;
-  if (lpb == 4) return launch_gen(general_collocate_kernel<4>, P, s);
-  if (lpb == 6) return launch_gen(general_collocate_kernel<6>, P, s);
+  count_launch(13, P.fp32);
+  if (lpb == 4)
+    return P.fp32 ? launch_gen(general_collocate_kernel<4, float>, P, s)
+                  : launch_gen(general_collocate_kernel<4, double>, P, s);
+  if (lpb == 6)
+    return P.fp32 ? launch_gen(general_collocate_kernel<6, float>, P, s)
+                  : launch_gen(general_collocate_kernel<6, double>, P, s);
   return hipErrorInvalidValue;
 }
 
@@ -535,6 +764,7 @@ hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipSt
   if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0 ||
       P.wp.np > lpb + 1)
     return hipErrorInvalidValue;
+  count_launch(13, P.fp32);
   auto go = [&](auto kernel) {
     const size_t lds = (size_t)P.wp.bytes;
     if (lds > 65536) {
@@ -546,20 +776,26 @@ hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipSt
                        P.units, P.items, P);
     return hipGetLastError();
   };
-  if (lpb == 4) return go(general_tile_kernel<4>);
-  if (lpb == 6) return go(general_tile_kernel<6>);
+  if (lpb == 4) return P.fp32 ? go(general_tile_kernel<4, float>) : go(general_tile_kernel<4, double>);
+  if (lpb == 6) return P.fp32 ? go(general_tile_kernel<6, float>) : go(general_tile_kernel<6, double>);
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_general_integrate(const PointParams &P, int lpb, hipStream_t s) {
+// General tasks with lp lo..hi: (v, v) for v <= 6, (0, 4), (5, 6), (7, 11).
+hipError_t launch_general_integrate(const PointParams &P, int lo, int hi, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
-  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.np > lpb + 1) return hipErrorInvalidValue;
-  if (lpb == 4)
-    return P.fp32 ? launch_gen(general_integrate_kernel<4, float>, P, s)
-                  : launch_gen(general_integrate_kernel<4, double>, P, s);
-  if (lpb == 6)
-    return P.fp32 ? launch_gen(general_integrate_kernel<6, float>, P, s)
-                  : launch_gen(general_integrate_kernel<6, double>, P, s);
+  if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.rows_cap < 1 || P.wp.j_cap < 1 ||
+      P.wp.np > hi + 1)
+    return hipErrorInvalidValue;
+  count_launch(13, P.fp32);
+  auto go = [&](auto kd, auto kf) { return P.fp32 ? launch_gen(kf, P, s) : launch_gen(kd, P, s); };
+#define GEN_INT_RANGE(A, B)                                                          \
+  if (lo == A && hi == B)                                                           \
+    return go(general_integrate_kernel<A, B, double>, general_integrate_kernel<A, B, float>);
+  GEN_INT_RANGE(0, 0) GEN_INT_RANGE(1, 1) GEN_INT_RANGE(2, 2) GEN_INT_RANGE(3, 3)
+  GEN_INT_RANGE(4, 4) GEN_INT_RANGE(5, 5) GEN_INT_RANGE(6, 6)
+  GEN_INT_RANGE(0, 4) GEN_INT_RANGE(5, 6) GEN_INT_RANGE(7, 11)
+#undef GEN_INT_RANGE
   return hipErrorInvalidValue;
 }
 

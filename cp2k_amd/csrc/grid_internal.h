@@ -113,6 +113,9 @@ struct LdsPlan {
   int bytes;
 };
 
+// Slots per level of the general change-of-variables matrices (lp 0..11).
+constexpr int kGmatSlots = 12;
+
 // Register-array bound of the point kernels for a maximum lp.
 inline int lpb_bucket(int lp) { return lp <= 4 ? 4 : lp <= 8 ? 8 : lp <= 12 ? 12 : 16; }
 
@@ -131,6 +134,10 @@ struct WavePlan {
   int by, bz;      // tile rows and planes (window = nx * by * bz)
   int o_cxyz, o_u; // doubles: cxyz [np^3]; {cab, alpha} (setup/finish) or tables [3][ext][np]
   int bytes;       // dynamic LDS per workgroup
+  // general-cell column walk (integrate), byte offsets in the wave region:
+  // ratio tables (doubles, q_cap), row table of a slab of planes (rows_cap
+  // ints), per row j its footprint (int4, j_cap) and column start (j_cap + 1)
+  int o_q, q_cap, o_rows, rows_cap, o_jinfo, o_cstart, j_cap;
 };
 
 struct PointParams {
@@ -164,7 +171,8 @@ struct PointParams {
   int fp32;               // FP32-mixed: point evaluation in FP32, accumulation in FP64
   // general cells: per level and lp the matrix M[o][i] of the change of
   // variables cxyz (i, packed (lx, ly, lz)) -> cijk (o, packed (il, jl, kl)),
-  // followed by its transpose; block (level, lp) at gmat + gmat_off[level * 8 + lp]
+  // followed by its transpose; block (level, lp) at
+  // gmat + gmat_off[level * kGmatSlots + lp]
   const double *gmat;
   const int *gmat_off;
 };

@@ -28,6 +28,7 @@
 #include "grid_device.h"
 
 namespace gm {
+void count_launch(int id, bool fp32);
 
 
 // Chunk bookkeeping in LDS ints.
@@ -899,6 +900,7 @@ hipError_t launch_collocate(const PointParams &P, bool ortho, hipStream_t s) {
   const int lp = P.plan.np - 1;
   if (lp > 16) return hipErrorInvalidValue;
   if (P.ntasks <= 0) return hipSuccess;
+  if (!P.count_only) count_launch(11, false);  // FP64 points (FP32 exp only, general cells)
   return ortho ? launch_collocate_kind<true>(P, lp, lds, s)
                : launch_collocate_kind<false>(P, lp, lds, s);
 }
@@ -918,6 +920,7 @@ hipError_t launch_integrate(const PointParams &P, bool ortho, hipStream_t s) {
   const int lp = P.plan.np - 1;
   if (lp > 16) return hipErrorInvalidValue;
   if (P.ntasks <= 0) return hipSuccess;
+  count_launch(11, false);  // FP64 points (FP32 exp only, general cells)
   const bool full = P.do_forces || P.compute_tau;
   if (ortho)
     return full ? launch_integrate_kind<true, true>(P, lp, lds, s)

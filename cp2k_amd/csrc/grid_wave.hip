@@ -37,6 +37,8 @@
 #include "grid_internal.h"
 
 namespace gm {
+void count_launch(int id, bool fp32);
+
 namespace {
 
 // Per-wave LDS region.
@@ -1114,6 +1116,7 @@ static hipError_t collocate_wave_r(const PointParams &P, int lpb, hipStream_t s)
 hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
+  count_launch(8, P.fp32);
   return P.fp32 ? collocate_wave_r<float>(P, lpb, s) : collocate_wave_r<double>(P, lpb, s);
 }
 
@@ -1133,6 +1136,7 @@ hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hip
   if (nunits <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0)
     return hipErrorInvalidValue;
+  count_launch(7, P.fp32);
   return P.fp32 ? collocate_tiles_r<float>(P, lpb, nunits, s)
                 : collocate_tiles_r<double>(P, lpb, nunits, s);
 }
@@ -1147,6 +1151,7 @@ static hipError_t integrate_wave_r(const PointParams &P, int lpb, hipStream_t s)
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
+  count_launch(9, P.fp32);
   return P.fp32 ? integrate_wave_r<float>(P, lpb, s) : integrate_wave_r<double>(P, lpb, s);
 }
 
@@ -1154,6 +1159,7 @@ hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0 || P.wp.np > 12)
     return hipErrorInvalidValue;
+  count_launch(9, false);  // FP64 in both modes
   return launch_wave(integrate_mfma_kernel, P, wave_blocks(P), s, P.tasks, P.levels);
 }
 

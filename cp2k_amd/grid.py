@@ -36,7 +36,8 @@ EXPORTS = [
     "grid_mi355x_task_points_by_lp", "grid_mi355x_task_counts",
     "grid_mi355x_create_batch", "grid_mi355x_free_batch", "grid_mi355x_batch_buffers",
     "grid_mi355x_batch_grid", "grid_mi355x_batch_run", "grid_mi355x_batch_task_points",
-    "grid_mi355x_autotune", "grid_mi355x_tuned_shape",
+    "grid_mi355x_autotune", "grid_mi355x_tuned_shape", "grid_mi355x_point_launches",
+    "grid_mi355x_collocate_task_list_noalloc",
 ]
 
 GRID_FUNC_AB = 100
@@ -94,6 +95,8 @@ def lib():
         L.offload_free_buffer.argtypes = [C.POINTER(OffloadBuffer)]
         L.grid_mi355x_kernel_time_ms.restype = C.c_double
         L.grid_mi355x_kernel_time_ms.argtypes = [C.c_int, C.POINTER(C.c_long)]
+        L.grid_mi355x_point_launches.argtypes = [C.c_int, C.POINTER(C.c_long),
+                                                  C.POINTER(C.c_long)]
         L.grid_mi355x_task_points.restype = C.c_longlong
         L.grid_mi355x_task_points.argtypes = [C.c_void_p]
         L.grid_mi355x_task_points_by_lp.restype = None
@@ -297,6 +300,13 @@ class GridTaskList:
         lib().grid_mi355x_collocate_task_list_dev(
             self.h, C.c_int(func), C.c_int(a.nlevels), C.c_void_p(pab.data_ptr()), gp)
 
+    def collocate_noalloc(self, func, pab, grids):
+        """grid_mi355x_collocate_task_list_noalloc on device buffers."""
+        a = self.args
+        gp = (C.c_void_p * a.nlevels)(*[g.data_ptr() for g in grids])
+        lib().grid_mi355x_collocate_task_list_noalloc(
+            self.h, C.c_int(func), C.c_int(a.nlevels), C.c_void_p(pab.data_ptr()), gp)
+
     def integrate_dev(self, grids, hab, pab=None, forces=None, virial=None,
                       compute_tau=False):
         a = self.args
@@ -335,6 +345,47 @@ class GridTaskList:
         return o.value, g.value, s.value
 
 
+class AbiBuffers:
+    """Persistent offload_buffers of one task list for repeated calls of the
+    host-buffer C-ABI (grid_collocate_task_list / grid_integrate_task_list),
+    as CP2K keeps its buffers between SCF steps: every call copies pab to the
+    device and grids / hab back to the host."""
+
+    def __init__(self, gt):
+        a = gt.args
+        self.gt = gt
+        self.pab = OffloadBuf(a.buffer_size)
+        self.hab = OffloadBuf(a.buffer_size)
+        self.rho = [OffloadBuf(int(np.prod(s))) for s in a.grid_shapes()]
+        self.pot = [OffloadBuf(int(np.prod(s))) for s in a.grid_shapes()]
+        self._rho = (C.POINTER(OffloadBuffer) * a.nlevels)(*[g.h for g in self.rho])
+        self._pot = (C.POINTER(OffloadBuffer) * a.nlevels)(*[g.h for g in self.pot])
+        self.forces = np.zeros((a.natoms, 3))
+        self.virial = np.zeros((3, 3))
+
+    def set_inputs(self, pab_blocks, V):
+        self.pab.host()[:] = pab_blocks
+        for b, v in zip(self.pot, V):
+            b.host()[:] = np.ascontiguousarray(v, dtype=np.float64).ravel()
+
+    def collocate(self, func):
+        a = self.gt.args
+        lib().grid_collocate_task_list(self.gt.h, C.c_int(func), C.c_int(a.nlevels),
+                                       _ptr(self.gt._npts, C.c_int), self.pab.h, self._rho)
+
+    def integrate(self, forces=False, virial=False, compute_tau=False):
+        a = self.gt.args
+        lib().grid_integrate_task_list(
+            self.gt.h, C.c_bool(compute_tau), C.c_int(a.natoms), C.c_int(a.nlevels),
+            _ptr(self.gt._npts, C.c_int), self.pab.h if (forces or virial) else None,
+            self._pot, self.hab.h, _ptr(self.forces) if forces else None,
+            _ptr(self.virial) if virial else None)
+
+    def free(self):
+        for b in [self.pab, self.hab] + self.rho + self.pot:
+            b.__del__()
+
+
 def set_stream(stream_handle):
     lib().grid_mi355x_set_stream(C.c_void_p(stream_handle or 0))
 
@@ -355,6 +406,14 @@ def kernel_time_ms(kernel_id):
     n = C.c_long()
     ms = lib().grid_mi355x_kernel_time_ms(C.c_int(kernel_id), C.byref(n))
     return ms, n.value
+
+
+def point_launches(kernel_id):
+    """(FP64, FP32) eager launches of the point kernels of kernel_id since
+    the last reset_timing()."""
+    a, b = C.c_long(), C.c_long()
+    lib().grid_mi355x_point_launches(C.c_int(kernel_id), C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 class _DevView:

@@ -78,6 +78,20 @@ void grid_mi355x_collocate_task_list_dev(const grid_task_list *task_list,
                                          const double *pab_blocks,
                                          double *const *grids);
 
+/* No counterpart in the reference.  grid_mi355x_collocate_task_list_dev for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): it calls no
+ * allocator.  Its per-call buffers (level table, decontracted density blocks,
+ * per-task coefficients) live in a static device array of 128 MiB that each
+ * device holds for the life of the process; a list needing more is refused
+ * with a message.  Calls that overlap on one device share the array and
+ * race.  Not for empty or batched lists. */
+void grid_mi355x_collocate_task_list_noalloc(const grid_task_list *task_list,
+                                             const enum grid_func func,
+                                             const int nlevels,
+                                             const double *pab_blocks,
+                                             double *const *grids);
+
 /* HBM-resident integrate: hab_blocks (hab_length doubles) is overwritten;
  * forces [natoms][3] and virial [3][3] are device pointers or NULL. */
 void grid_mi355x_integrate_task_list_dev(
@@ -110,6 +124,12 @@ void grid_mi355x_set_timing(const bool enable);
 void grid_mi355x_reset_timing(void);
 /* Returns total milliseconds and number of launches of kernel id. */
 double grid_mi355x_kernel_time_ms(const int kernel_id, long *launches);
+
+/* Eager launches of the point kernels of kernel id (ids as above: 7, 8, 9,
+ * 11, 13) since the last grid_mi355x_reset_timing, split by the arithmetic
+ * of their per-point evaluation: FP64 or FP32 (FP32-mixed mode).  Kernels
+ * replayed from a batch graph are not counted. */
+void grid_mi355x_point_launches(const int kernel_id, long *fp64, long *fp32);
 
 /* Number of (task, grid point) pairs the list visits, counted on the device
  * with the same membership rules as collocate/integrate. */

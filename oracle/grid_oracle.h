@@ -14,11 +14,13 @@
  * of the reference's multiplicative recurrences, which changes results only
  * at the level of rounding.
  *
- * Pinning (see tests/test_oracle_golden.py and tests/test_oracle_ref.py):
+ * Pinning (tests/test_oracle.py; tests/brute.py is the analytic evaluator):
  *   - single-task collocate/integrate/forces/virial, ortho + general,
  *     border masks, wrap-around, tau: the 13 sample_tasks known-answer files;
- *   - all 35 grid_func stencils, random l/exponents/geometry: the reference's
- *     own single-task sources compiled into oracle/_ref (when present);
+ *   - all 35 grid_func stencils and the hab/force/virial getters: a brute-
+ *     force analytic evaluation of the Gaussian products and derivatives;
+ *   - truncation regions, random l/exponents/geometry: the reference's own
+ *     single-task sources compiled into oracle/_ref (when present);
  *   - task-list layer (sorting, load_pab/store_hab contraction, rscale,
  *     force scaling): batched replay of the fixtures with the reference's
  *     dummy-task-list conventions, a textbook triple-loop contraction, and the

@@ -153,3 +153,28 @@ def test_empty_task_list_needs_no_device(L):
     r = _run(code)
     assert r.returncode == 0, r.stderr
     assert "OK" in r.stdout
+
+
+@pytest.mark.parametrize("compiler", [["gcc", "-std=c11", "-x", "c"],
+                                      ["g++", "-std=c++17", "-x", "c++"]])
+def test_headers_compile_and_link_as_a_caller(L, tmp_path, compiler):
+    """A C (and C++) translation unit that includes every header and takes
+    the address of every declared function compiles warning-free and links
+    against the library: the declarations are usable as written."""
+    from cp2k_amd import build
+    names = declared_functions()
+    headers = sorted(h.name for h in (ROOT / "include").glob("*.h"))
+    src = "\n".join(f'#include "{h}"' for h in headers)
+    src += "\n#include <stdio.h>\nint main(void) {\n  void *fns[] = {\n"
+    src += ",\n".join(f"    (void *)&{n}" for n in names)
+    src += "\n  };\n  printf(\"%d\\n\", (int)(sizeof fns / sizeof fns[0]));\n  return 0;\n}\n"
+    f = tmp_path / "caller.src"
+    f.write_text(src)
+    exe = tmp_path / "caller"
+    cmd = compiler + [str(f), "-x", "none", "-Wall", "-Wextra", "-Werror",
+                      f"-I{ROOT / 'include'}", "-o", str(exe), str(build.LIB),
+                      f"-Wl,-rpath,{build.LIB.parent}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 0 and int(run.stdout) == len(names), run.stderr

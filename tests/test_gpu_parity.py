@@ -104,14 +104,29 @@ def test_config_integrate_hab_only(built):
     assert replay.rel_err(h1, h2) < tol, name
 
 
-@pytest.mark.parametrize("func", [200, 302, 412, 433, 501, 603, 701, 803, 902, 905, 1003])
-def test_funcs(func):
-    print("func", func, flush=True)
-    args, pab, V = synth.config("h2o1", seed=func, n=24, flip=0.5)
-    got = grid.GridTaskList(args).collocate(func, pab)
-    want = O.TaskList(args).collocate(func, pab)
+# every grid_func of src/grid/common/grid_constants.h:14-48, on an
+# orthorhombic list (wave-per-task and tile kernels, per-task kernels for
+# lp > 8) and on a triclinic list (general kernels, per-task for lp > 6)
+ALL_FUNCS = [100, 200, 301, 302, 303, 411, 412, 413, 421, 422, 423, 431, 432, 433,
+             501, 502, 503, 601, 602, 603, 701, 702, 703, 801, 802, 803,
+             901, 902, 903, 904, 905, 906, 1001, 1002, 1003]
+
+
+@pytest.fixture(scope="module", params=["h2o1", "tri6"])
+def func_list(request):
+    kw = dict(n=24) if request.param == "h2o1" else {}
+    args, pab, V = synth.config(request.param, seed=17, flip=0.5, **kw)
+    return request.param, args, pab, grid.GridTaskList(args), O.TaskList(args)
+
+
+@pytest.mark.parametrize("func", ALL_FUNCS)
+def test_funcs(func_list, func):
+    name, args, pab, gt, ot = func_list
+    print("func", name, func, flush=True)
+    got = gt.collocate(func, pab)
+    want = ot.collocate(func, pab)
     for g, w in zip(got, want):
-        assert replay.rel_err(g, w) < TOL
+        assert replay.rel_err(g, w) < TOL, (name, func)
 
 
 def test_device_api_matches_host_api():
