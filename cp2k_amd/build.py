@@ -17,7 +17,7 @@ LIB = PKG / "libcp2k_grid_mi355x.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = [CSRC / "grid_api.cpp", CSRC / "grid_kernels.hip", CSRC / "grid_wave.hip",
-           CSRC / "grid_general.hip", CSRC / "grid_noalloc.hip"]
+           CSRC / "grid_general.hip", CSRC / "grid_noalloc.hip", CSRC / "grid_buffers.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-munsafe-fp-atomics",

@@ -1,7 +1,8 @@
 // Host side of the MI355X grid backend: the C-ABI of src/grid/grid_task_list.h,
 // src/grid/common/grid_library.h, src/grid/common/grid_basis_set.h and
 // src/offload/offload_buffer.h, plus the backend-level grid_gpu_* symbols and
-// the extensions in include/grid_mi355x.h.
+// the extensions in include/grid_mi355x.h, and the single-task entry points of
+// include/grid_pgf_product.h.
 //
 // Compiled with -ffp-contract=off: the task geometry computed here (cube
 // centres, discretised radii, index boxes) must round exactly like the
@@ -22,8 +23,10 @@
 #include <numeric>
 #include <vector>
 
+#include "../../include/grid_buffers.h"
 #include "../../include/grid_library.h"
 #include "../../include/grid_mi355x.h"
+#include "../../include/grid_pgf_product.h"
 #include "../../include/grid_task_list.h"
 #include "grid_internal.h"
 
@@ -58,6 +61,10 @@ hipError_t launch_general_integrate(const PointParams &P, int lo, int hi, hipStr
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 double *noalloc_scratch(long long *doubles, hipError_t *err);
 hipError_t launch_write_levels(const Level *h, int n, Level *dst, hipStream_t s);
+hipError_t launch_pack(const double *const *blocks, const int *sizes, const long long *offsets,
+                       int npairs, double *dest, hipStream_t s);
+hipError_t launch_unpack(double *const *blocks, const int *sizes, const long long *offsets,
+                         int npairs, const double *src, hipStream_t s);
 }  // namespace gm
 
 using namespace gm;
@@ -495,6 +502,80 @@ void upload(DevBuf &b, const std::vector<T> &v) {
 int lds_limit();
 TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general = false);
 
+// Geometry of a task from ra, rab, the exponents and the radius (FP64, in the
+// reference's operation order): kTaskSkip if the task maps to no point
+// (2 radius < the largest |dh| entry), kTaskOrtho for the orthorhombic path
+// (ortho_cxyz_to_grid) or 0 for the general path (general_cijk_to_grid); the
+// record's flags are set accordingly.  `num` names the task in messages.
+int task_geometry(Task &T, const Layout &y, bool ortho, int bm, int num) {
+  // cab_to_grid, src/grid/ref/grid_ref_collint.h:928-947
+  double hmax = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) hmax = fmax(hmax, fabs(y.dh[i][j]));
+  T.zetp = T.zeta + T.zetb;
+  const double f = T.zetb / T.zetp;
+  const double rab2 = T.rab[0] * T.rab[0] + T.rab[1] * T.rab[1] + T.rab[2] * T.rab[2];
+  T.pexp = exp(-T.zeta * f * rab2);
+  for (int d = 0; d < 3; d++) {
+    T.rp[d] = T.ra[d] + f * T.rab[d];
+    T.rb[d] = T.ra[d] + T.rab[d];
+  }
+  if (2.0 * T.radius < hmax) {
+    T.flags = kTaskSkip;
+    return kTaskSkip;
+  }
+  if (ortho) {
+    // ortho_cxyz_to_grid, grid_ref_collint.h:222-257
+    T.flags = kTaskOrtho;
+    for (int d = 0; d < 3; d++) {
+      double s2 = 0.0;
+      for (int e = 0; e < 3; e++) s2 += y.dh_inv[e][d] * T.rp[e];
+      T.center[d] = (int)floor(s2);
+    }
+    for (int d = 0; d < 3; d++) T.roff[d] = T.rp[d] - ((double)T.center[d]) * y.dh[d][d];
+    const double drmin = fmin(y.dh[0][0], fmin(y.dh[1][1], y.dh[2][2]));
+    T.disr = drmin * fmax(1.0, ceil(T.radius / drmin));
+    T.cmax = 0;
+    for (int d = 0; d < 3; d++) {
+      T.lo[d] = (int)ceil(-1e-8 - T.disr * y.dh_inv[d][d]);
+      T.hi[d] = 1 - T.lo[d];
+      if (y.npts_global[d] != y.npts_local[d]) {
+        // non-periodic direction: the cube must fit without wrapping (:247-253)
+        const int off = pmod(T.center[d] + T.lo[d] - y.shift_local[d], y.npts_global[d]) - T.lo[d];
+        if (!(off + T.hi[d] < y.npts_local[d]) || !(off + T.lo[d] >= 0))
+          fatal("task %d: cube does not fit into the non-periodic local grid", num);
+      }
+      T.cmax = std::max(T.cmax, T.hi[d]);
+    }
+    return kTaskOrtho;
+  }
+  // general_cijk_to_grid, grid_ref_collint.h:591-654
+  T.flags = 0;
+  for (int d = 0; d < 3; d++) {
+    T.blo[d] = 0;
+    T.bhi[d] = y.npts_local[d] - 1;
+    if (bm & (1 << (2 * d))) T.blo[d] += y.border_width[d];
+    if (bm & (1 << (2 * d + 1))) T.bhi[d] -= y.border_width[d];
+    T.gp[d] = 0.0;
+    for (int e = 0; e < 3; e++) T.gp[d] += y.dh_inv[e][d] * T.rp[e];
+    T.lo[d] = 2147483647;
+    T.hi[d] = -2147483647 - 1;
+  }
+  for (int si = -1; si <= 1; si++)
+    for (int sj = -1; sj <= 1; sj++)
+      for (int sk = -1; sk <= 1; sk++) {
+        const double x = T.rp[0] + si * T.radius;
+        const double yy = T.rp[1] + sj * T.radius;
+        const double z = T.rp[2] + sk * T.radius;
+        for (int d = 0; d < 3; d++) {
+          const double r = y.dh_inv[0][d] * x + y.dh_inv[1][d] * yy + y.dh_inv[2][d] * z;
+          T.lo[d] = std::min(T.lo[d], (int)floor(r));
+          T.hi[d] = std::max(T.hi[d], (int)ceil(r));
+        }
+      }
+  return 0;
+}
+
 void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevels,
                      int natoms, int nkinds, int nblocks, const int *block_offsets,
                      const double *atom_positions, const int *atom_kinds,
@@ -748,50 +829,18 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
       T.ra[d] = atom_positions[3 * ia + d];
       T.rab[d] = rab_list[3 * t + d];
     }
-    // cab_to_grid, src/grid/ref/grid_ref_collint.h:928-947
-    double hmax = 0.0;
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) hmax = fmax(hmax, fabs(y.dh[i][j]));
-    T.zetp = T.zeta + T.zetb;
-    const double f = T.zetb / T.zetp;
-    const double rab2 = T.rab[0] * T.rab[0] + T.rab[1] * T.rab[1] + T.rab[2] * T.rab[2];
-    T.pexp = exp(-T.zeta * f * rab2);
-    for (int d = 0; d < 3; d++) {
-      T.rp[d] = T.ra[d] + f * T.rab[d];
-      T.rb[d] = T.ra[d] + T.rab[d];
-    }
     const int lp0 = T.la_max + T.lb_max;
     tl->max_la = std::max(tl->max_la, T.la_max);
     tl->max_lb = std::max(tl->max_lb, T.lb_max);
     tl->max_lp = std::max(tl->max_lp, lp0);
-    if (2.0 * T.radius < hmax) {
-      T.flags = kTaskSkip;
+    const int kind =
+        task_geometry(T, y, (level_ortho ? level_ortho[level] != 0 : orthorhombic) && bm == 0, bm,
+                      t + 1);
+    if (kind == kTaskSkip) {
       tl->n_skip++;
       continue;
     }
-    if ((level_ortho ? level_ortho[level] != 0 : orthorhombic) && bm == 0) {
-      // ortho_cxyz_to_grid, grid_ref_collint.h:222-257
-      T.flags = kTaskOrtho;
-      for (int d = 0; d < 3; d++) {
-        double s2 = 0.0;
-        for (int e = 0; e < 3; e++) s2 += y.dh_inv[e][d] * T.rp[e];
-        T.center[d] = (int)floor(s2);
-      }
-      for (int d = 0; d < 3; d++) T.roff[d] = T.rp[d] - ((double)T.center[d]) * y.dh[d][d];
-      const double drmin = fmin(y.dh[0][0], fmin(y.dh[1][1], y.dh[2][2]));
-      T.disr = drmin * fmax(1.0, ceil(T.radius / drmin));
-      T.cmax = 0;
-      for (int d = 0; d < 3; d++) {
-        T.lo[d] = (int)ceil(-1e-8 - T.disr * y.dh_inv[d][d]);
-        T.hi[d] = 1 - T.lo[d];
-        if (y.npts_global[d] != y.npts_local[d]) {
-          // non-periodic direction: the cube must fit without wrapping (:247-253)
-          const int off = pmod(T.center[d] + T.lo[d] - y.shift_local[d], y.npts_global[d]) - T.lo[d];
-          if (!(off + T.hi[d] < y.npts_local[d]) || !(off + T.lo[d] >= 0))
-            fatal("task %d: cube does not fit into the non-periodic local grid", t + 1);
-        }
-        T.cmax = std::max(T.cmax, T.hi[d]);
-      }
+    if (kind == kTaskOrtho) {
       tl->has_ortho = true;
       tl->max_cmax = std::max(tl->max_cmax, T.cmax);
       tl->max_rows_ortho = std::max(tl->max_rows_ortho, 2 - 2 * T.lo[1]);
@@ -799,29 +848,6 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
       tl->n_ortho++;
       tl->hist[1][std::min(lp0, kCounterLp - 1)]++;
     } else {
-      // general_cijk_to_grid, grid_ref_collint.h:591-654
-      for (int d = 0; d < 3; d++) {
-        T.blo[d] = 0;
-        T.bhi[d] = y.npts_local[d] - 1;
-        if (bm & (1 << (2 * d))) T.blo[d] += y.border_width[d];
-        if (bm & (1 << (2 * d + 1))) T.bhi[d] -= y.border_width[d];
-        T.gp[d] = 0.0;
-        for (int e = 0; e < 3; e++) T.gp[d] += y.dh_inv[e][d] * T.rp[e];
-        T.lo[d] = 2147483647;
-        T.hi[d] = -2147483647 - 1;
-      }
-      for (int si = -1; si <= 1; si++)
-        for (int sj = -1; sj <= 1; sj++)
-          for (int sk = -1; sk <= 1; sk++) {
-            const double x = T.rp[0] + si * T.radius;
-            const double yy = T.rp[1] + sj * T.radius;
-            const double z = T.rp[2] + sk * T.radius;
-            for (int d = 0; d < 3; d++) {
-              const double r = y.dh_inv[0][d] * x + y.dh_inv[1][d] * yy + y.dh_inv[2][d] * z;
-              T.lo[d] = std::min(T.lo[d], (int)floor(r));
-              T.hi[d] = std::max(T.hi[d], (int)ceil(r));
-            }
-          }
       tl->has_general = true;
       for (int d = 0; d < 3; d++) tl->max_range = std::max(tl->max_range, T.hi[d] - T.lo[d] + 1);
       tl->max_rows_general = std::max(tl->max_rows_general, T.hi[1] - T.lo[1] + 1);
@@ -2147,6 +2173,73 @@ double *device_mirror(const offload_buffer *b, DevBuf &scratch) {
   if (b->device_buffer) return b->device_buffer;
   return (double *)scratch.get(b->size ? b->size : sizeof(double));
 }
+
+// Residency of offload_buffers (extension, include/grid_buffers.h): which
+// copy holds the current data, and whether results the grid calls write into
+// the buffer stay in HBM.  Buffers never named by the extension keep the
+// reference behaviour: inputs are copied host -> device on every call and
+// results device -> host.
+struct Residency {
+  bool resident = false;        // results stay on the device
+  bool device_current = false;  // device_buffer holds the current data
+};
+std::mutex res_mu;
+std::map<const offload_buffer *, Residency> res_tab;
+
+Residency residency_of(const offload_buffer *b) {
+  std::lock_guard<std::mutex> g(res_mu);
+  auto it = res_tab.find(b);
+  return it == res_tab.end() ? Residency{} : it->second;
+}
+
+void set_device_current(const offload_buffer *b, bool v) {
+  std::lock_guard<std::mutex> g(res_mu);
+  auto it = res_tab.find(b);
+  if (it != res_tab.end()) {
+    it->second.device_current = v;
+  } else if (v) {
+    res_tab[b].device_current = true;
+  }
+}
+
+void forget_buffer(const offload_buffer *b) {
+  std::lock_guard<std::mutex> g(res_mu);
+  res_tab.erase(b);
+}
+
+// Device view of an input buffer: its device copy when that is current,
+// else `bytes` copied host -> device into the buffer's mirror.
+const double *input_view(const offload_buffer *b, DevBuf &scratch, size_t bytes, hipStream_t s) {
+  if (b->device_buffer && residency_of(b).device_current) return b->device_buffer;
+  double *d = device_mirror(b, scratch);
+  HIP_CHECK(hipMemcpyAsync(d, b->host_buffer, bytes, hipMemcpyHostToDevice, s));
+  return d;
+}
+
+// After a grid call wrote `bytes` of results at d (the buffer's device view):
+// a resident buffer keeps them in HBM, any other gets them copied to host.
+void output_done(offload_buffer *b, const double *d, size_t bytes, hipStream_t s) {
+  if (b->device_buffer == d && residency_of(b).resident) {
+    set_device_current(b, true);
+    return;
+  }
+  HIP_CHECK(hipMemcpyAsync(b->host_buffer, d, bytes, hipMemcpyDeviceToHost, s));
+  set_device_current(b, false);
+}
+
+// Zeroes a buffer the way its residency says (empty task lists: no device
+// work unless the buffer is resident).
+void zero_output(offload_buffer *b) {
+  if (b->device_buffer && residency_of(b).resident) {
+    const hipStream_t s = stream();
+    HIP_CHECK(launch_fill(b->device_buffer, (long long)(b->size / sizeof(double)), 0.0, s));
+    sync_stream(s);
+    set_device_current(b, true);
+    return;
+  }
+  memset(b->host_buffer, 0, b->size);
+  set_device_current(b, false);
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2243,6 +2336,153 @@ void batch_enqueue(BatchImpl *b, int what, int func, bool tau, bool forces,
   tl->static_levels = nullptr;
 }
 
+// ---------------------------------------------------------------------------
+// Single-task entry points: the drop-ins for grid_cpu_collocate_pgf_product /
+// grid_cpu_integrate_pgf_product, which src/grid/grid_api.F:110-490
+// (collocate_pgf_product / integrate_pgf_product) binds.  One task runs on
+// the per-task kernels; every array may live in host or device memory.
+// ---------------------------------------------------------------------------
+struct PgfContext {
+  std::mutex mu;
+  TaskListImpl tl;  // holds the maxima of the one task (LDS plan) only
+  DevBuf rec, grid, pab, hab, out, hdab, hadb, a_hdab;
+};
+
+PgfContext &pgf_ctx() {
+  static PgfContext *c = new PgfContext();  // lives until the process ends
+  return *c;
+}
+
+bool device_pointer(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory on some runtimes
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Device view of n doubles at p: p itself if it is device memory, else a
+// copy in `mirror` (copied in when copy_in).
+double *device_view(const double *p, size_t n, DevBuf &mirror, bool copy_in, hipStream_t s) {
+  if (!p || device_pointer(p)) return const_cast<double *>(p);
+  double *d = (double *)mirror.get(sizeof(double) * std::max<size_t>(n, 1));
+  if (copy_in) HIP_CHECK(hipMemcpyAsync(d, p, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  return d;
+}
+
+void copy_back(double *dst, const double *view, size_t n, hipStream_t s) {
+  if (dst && view != dst)
+    HIP_CHECK(hipMemcpyAsync(dst, view, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+}
+
+// dst[0..n) += src[0..n) (src on the host; dst host or device).
+void add_into(double *dst, const double *src, int n, hipStream_t s) {
+  if (!dst) return;
+  double tmp[24];
+  if (device_pointer(dst)) {
+    HIP_CHECK(hipMemcpyAsync(tmp, dst, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) tmp[i] += src[i];
+    HIP_CHECK(hipMemcpyAsync(dst, tmp, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    for (int i = 0; i < n; i++) dst[i] += src[i];
+  }
+}
+
+struct PgfArgs {
+  bool orthorhombic;
+  int border_mask, la_max, la_min, lb_max, lb_min;
+  double zeta, zetb, rscale;
+  const double (*dh)[3], (*dh_inv)[3];
+  const double *ra, *rab;
+  const int *npts_global, *npts_local, *shift_local, *border_width;
+  double radius;
+  int o1, o2, n1, n2;
+};
+
+// Task, group and level records of the one task (uploaded to c.rec), the
+// task's kind (kTaskSkip, kTaskOrtho or general) and the LDS-plan maxima.
+int pgf_records(PgfContext &c, const PgfArgs &a, Level &L, hipStream_t s, const Task **dtask,
+                const Group **dgroup, const Level **dlevel) {
+  if (a.la_max < 0 || a.lb_max < 0 || a.la_min < 0 || a.lb_min < 0 || a.la_min > a.la_max ||
+      a.lb_min > a.lb_max)
+    fatal("pgf_product: bad l range a %d..%d, b %d..%d", a.la_min, a.la_max, a.lb_min, a.lb_max);
+  if (a.o1 < 0 || a.o2 < 0 || a.o1 + ncoset(a.la_max) > a.n1 || a.o2 + ncoset(a.lb_max) > a.n2)
+    fatal("pgf_product: block [%d][%d] does not hold offsets o1=%d, o2=%d for la_max=%d, "
+          "lb_max=%d", a.n2, a.n1, a.o1, a.o2, a.la_max, a.lb_max);
+  Layout &y = L.lay;
+  for (int i = 0; i < 3; i++) {
+    y.npts_global[i] = a.npts_global[i];
+    y.npts_local[i] = a.npts_local[i];
+    y.shift_local[i] = a.shift_local[i];
+    y.border_width[i] = a.border_width[i];
+    if (y.npts_local[i] <= 0 || y.npts_global[i] <= 0) fatal("pgf_product: non-positive grid size");
+    for (int j = 0; j < 3; j++) {
+      y.dh[i][j] = a.dh[i][j];
+      y.dh_inv[i][j] = a.dh_inv[i][j];
+    }
+  }
+  L.npts = (long long)y.npts_local[0] * y.npts_local[1] * y.npts_local[2];
+  L.gridf = nullptr;
+  Task T;
+  memset(&T, 0, sizeof T);
+  T.la_max = a.la_max;
+  T.la_min = a.la_min;
+  T.lb_max = a.lb_max;
+  T.lb_min = a.lb_min;
+  T.o1 = a.o1;
+  T.o2 = a.o2;
+  T.zeta = a.zeta;
+  T.zetb = a.zetb;
+  T.rscale = a.rscale;
+  T.radius = a.radius;
+  T.geo = -1;
+  for (int d = 0; d < 3; d++) {
+    T.ra[d] = a.ra[d];
+    T.rab[d] = a.rab[d];
+  }
+  const int kind = task_geometry(T, y, a.orthorhombic && a.border_mask == 0, a.border_mask, 1);
+  TaskListImpl &tl = c.tl;
+  tl.max_la = a.la_max;
+  tl.max_lb = a.lb_max;
+  tl.max_lp = a.la_max + a.lb_max;
+  tl.has_ortho = kind == kTaskOrtho;
+  tl.has_general = kind == 0;
+  tl.max_cmax = tl.max_rows_ortho = tl.max_range = tl.max_rows_general = tl.max_planes = 0;
+  if (kind == kTaskOrtho) {
+    tl.max_cmax = T.cmax;
+    tl.max_rows_ortho = 2 - 2 * T.lo[1];
+    tl.max_planes = 2 - 2 * T.lo[2];
+  } else if (kind == 0) {
+    for (int d = 0; d < 3; d++) tl.max_range = std::max(tl.max_range, T.hi[d] - T.lo[d] + 1);
+    tl.max_rows_general = T.hi[1] - T.lo[1] + 1;
+    tl.max_planes = T.hi[2] - T.lo[2] + 1;
+  }
+  Group G;
+  memset(&G, 0, sizeof G);
+  G.ncoa = a.n1;
+  G.ncob = a.n2;
+  struct alignas(16) Rec {
+    Task t;
+    Group g;
+    Level l;
+  } r;
+  r.t = T;
+  r.g = G;
+  r.l = L;
+  Rec *d = (Rec *)c.rec.get(sizeof(Rec));
+  // every single-task call ends synchronised, so no kernel still reads d
+  HIP_CHECK(hipMemcpy(d, &r, sizeof(Rec), hipMemcpyHostToDevice));
+  (void)s;
+  *dtask = &d->t;
+  *dgroup = &d->g;
+  *dlevel = &d->l;
+  return kind;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2260,6 +2500,7 @@ void offload_create_buffer(const int length, offload_buffer **buffer) {
       return;
     }
     if (requested <= (*buffer)->size) return;
+    set_device_current(*buffer, false);  // new, uninitialised memory
     if ((*buffer)->host_buffer) (void)hipHostFree((*buffer)->host_buffer);
     if ((*buffer)->device_buffer) (void)hipFree((*buffer)->device_buffer);
   } else {
@@ -2277,6 +2518,7 @@ void offload_create_buffer(const int length, offload_buffer **buffer) {
 
 void offload_free_buffer(offload_buffer *buffer) {
   if (!buffer) return;
+  forget_buffer(buffer);
   if (buffer->host_buffer) (void)hipHostFree(buffer->host_buffer);
   if (buffer->device_buffer) (void)hipFree(buffer->device_buffer);
   free(buffer);
@@ -2445,7 +2687,7 @@ void grid_collocate_task_list(const grid_task_list *task_list, const enum grid_f
   if (!task_list) return;
   TaskListImpl *tl = checked(task_list);
   if (tl->empty) {  // src/grid/grid_task_list.c:185-189
-    for (int l = 0; l < nlevels; l++) memset(grids[l]->host_buffer, 0, grids[l]->size);
+    for (int l = 0; l < nlevels; l++) zero_output(grids[l]);
     return;
   }
   check_npts(tl, nlevels, (const int *)npts_local);
@@ -2460,7 +2702,7 @@ void grid_integrate_task_list(const grid_task_list *task_list, const bool comput
   if (!task_list) return;
   TaskListImpl *tl = checked(task_list);
   if (tl->empty) {  // src/grid/grid_task_list.c:290-311
-    memset(hab_blocks->host_buffer, 0, hab_blocks->size);
+    zero_output(hab_blocks);
     if (virial) memset(virial, 0, 9 * sizeof(double));
     if (forces) memset(forces, 0, 3 * sizeof(double) * (size_t)natoms);
     return;
@@ -2510,15 +2752,13 @@ void grid_gpu_collocate_task_list(const grid_gpu_task_list *ptr, const enum grid
                                   offload_buffer **grids) {
   TaskListImpl *tl = checked(ptr);
   if (tl->empty) {
-    for (int l = 0; l < nlevels; l++) memset(grids[l]->host_buffer, 0, grids[l]->size);
+    for (int l = 0; l < nlevels; l++) zero_output(grids[l]);
     return;
   }
   if (nlevels != tl->nlevels) fatal("nlevels does not match the task list");
   const hipStream_t s = stream();
   check_buffer(tl, pab_blocks, "pab_blocks");
-  double *pab = device_mirror(pab_blocks, tl->pab_dev);
-  HIP_CHECK(hipMemcpyAsync(pab, pab_blocks->host_buffer, pab_blocks->size,
-                           hipMemcpyHostToDevice, s));
+  const double *pab = input_view(pab_blocks, tl->pab_dev, pab_blocks->size, s);
   std::vector<double *> g(nlevels);
   for (int l = 0; l < nlevels; l++) {
     const size_t need = sizeof(double) * (size_t)tl->npts_local[3 * l] *
@@ -2527,9 +2767,7 @@ void grid_gpu_collocate_task_list(const grid_gpu_task_list *ptr, const enum grid
     g[l] = device_mirror(grids[l], tl->grid_dev[l]);
   }
   run_collocate(tl, func, pab, g.data(), s);
-  for (int l = 0; l < nlevels; l++)
-    HIP_CHECK(hipMemcpyAsync(grids[l]->host_buffer, g[l], grids[l]->size,
-                             hipMemcpyDeviceToHost, s));
+  for (int l = 0; l < nlevels; l++) output_done(grids[l], g[l], grids[l]->size, s);
   sync_stream(s);
 }
 
@@ -2539,7 +2777,7 @@ void grid_gpu_integrate_task_list(const grid_gpu_task_list *ptr, const bool comp
                                   double *forces, double *virial) {
   TaskListImpl *tl = checked(ptr);
   if (tl->empty) {
-    memset(hab_blocks->host_buffer, 0, hab_blocks->size);
+    zero_output(hab_blocks);
     if (virial) memset(virial, 0, 9 * sizeof(double));
     if (forces) memset(forces, 0, 3 * sizeof(double) * (size_t)tl->natoms);
     return;
@@ -2551,26 +2789,20 @@ void grid_gpu_integrate_task_list(const grid_gpu_task_list *ptr, const bool comp
     const size_t need = sizeof(double) * (size_t)tl->npts_local[3 * l] *
                         tl->npts_local[3 * l + 1] * tl->npts_local[3 * l + 2];
     if (grids[l]->size < need) fatal("grid buffer of level %d is too small", l + 1);
-    double *d = device_mirror(grids[l], tl->grid_dev[l]);
-    HIP_CHECK(hipMemcpyAsync(d, grids[l]->host_buffer, need, hipMemcpyHostToDevice, s));
-    g[l] = d;
+    g[l] = input_view(grids[l], tl->grid_dev[l], need, s);
   }
   const double *pab = nullptr;
   check_buffer(tl, hab_blocks, "hab_blocks");
   if (pab_blocks && (forces || virial)) {
     check_buffer(tl, pab_blocks, "pab_blocks");
-    double *d = device_mirror(pab_blocks, tl->pab_dev);
-    HIP_CHECK(hipMemcpyAsync(d, pab_blocks->host_buffer, pab_blocks->size,
-                             hipMemcpyHostToDevice, s));
-    pab = d;
+    pab = input_view(pab_blocks, tl->pab_dev, pab_blocks->size, s);
   }
   double *hab = device_mirror(hab_blocks, tl->hab_dev);
   const long long hab_len = (long long)(hab_blocks->size / sizeof(double));
   double *fv = (double *)tl->forces_dev.get(sizeof(double) * (3 * (size_t)tl->natoms + 9));
   run_integrate(tl, compute_tau, pab, g.data(), hab, hab_len, forces ? fv : nullptr,
                 virial ? fv + 3 * tl->natoms : nullptr, s);
-  HIP_CHECK(hipMemcpyAsync(hab_blocks->host_buffer, hab, hab_blocks->size,
-                           hipMemcpyDeviceToHost, s));
+  output_done(hab_blocks, hab, hab_blocks->size, s);
   if (forces)
     HIP_CHECK(hipMemcpyAsync(forces, fv, sizeof(double) * 3 * tl->natoms,
                              hipMemcpyDeviceToHost, s));
@@ -2679,6 +2911,229 @@ void grid_mi355x_integrate_task_list_dev(const grid_task_list *task_list, const 
     fatal("hab_blocks holds %ld doubles but the task list addresses %lld", hab_length,
           tl->block_end);
   run_integrate(tl, compute_tau, pab_blocks, grids, hab_blocks, hab_length, forces, virial, s);
+}
+
+// ---- single task, src/grid/cpu/grid_cpu_collocate.h / grid_cpu_integrate.h
+//      (bound by src/grid/grid_api.F:138-172, 311-350) ----
+void grid_mi355x_collocate_pgf_product(
+    const bool orthorhombic, const int border_mask, const enum grid_func func, const int la_max,
+    const int la_min, const int lb_max, const int lb_min, const double zeta, const double zetb,
+    const double rscale, const double dh[3][3], const double dh_inv[3][3], const double ra[3],
+    const double rab[3], const int npts_global[3], const int npts_local[3],
+    const int shift_local[3], const int border_width[3], const double radius, const int o1,
+    const int o2, const int n1, const int n2, const double *pab, double *grid) {
+  ensure_device();
+  PgfContext &c = pgf_ctx();
+  std::lock_guard<std::mutex> g(c.mu);
+  int ld[4];
+  if (!func_ldiffs(func, ld)) fatal("unknown grid_func %d", (int)func);
+  const hipStream_t s = stream();
+  const PgfArgs a{orthorhombic, border_mask, la_max, la_min, lb_max, lb_min, zeta, zetb, rscale,
+                  dh, dh_inv, ra, rab, npts_global, npts_local, shift_local, border_width,
+                  radius, o1, o2, n1, n2};
+  Level L;
+  const size_t npts = (size_t)npts_local[0] * npts_local[1] * npts_local[2];
+  double *gd = device_view(grid, npts, c.grid, true, s);
+  L.grid = gd;
+  const Task *dt;
+  const Group *dg;
+  const Level *dl;
+  const int kind = pgf_records(c, a, L, s, &dt, &dg, &dl);
+  if (kind != kTaskSkip) {
+    PointParams P{};
+    P.tasks = dt;
+    P.ntasks = 1;
+    P.groups = dg;
+    P.levels = dl;
+    P.cart = device_view(pab, (size_t)n1 * n2, c.pab, true, s);
+    P.cart_len = (long long)n1 * n2;
+    P.func = func;
+    for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
+    P.fp32 = precision() == GRID_MI355X_FP32_MIXED;
+    P.plan = make_plan(&c.tl, ld, false);
+    HIP_CHECK(launch_collocate(P, kind == kTaskOrtho, s));
+  }
+  copy_back(grid, gd, npts, s);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void grid_mi355x_integrate_pgf_product(
+    const bool orthorhombic, const bool compute_tau, const int border_mask, const int la_max,
+    const int la_min, const int lb_max, const int lb_min, const double zeta, const double zetb,
+    const double dh[3][3], const double dh_inv[3][3], const double ra[3], const double rab[3],
+    const int npts_global[3], const int npts_local[3], const int shift_local[3],
+    const int border_width[3], const double radius, const int o1, const int o2, const int n1,
+    const int n2, const double *grid, double *hab, const double *pab, double *forces,
+    double *virials, double *hdab, double *hadb, double *a_hdab) {
+  // src/grid/ref/grid_ref_integrate.c:56-60
+  const bool calc_forces = forces != nullptr || hdab != nullptr;
+  const bool calc_virial = virials != nullptr || a_hdab != nullptr;
+  if (calc_virial && !calc_forces) fatal("integrate_pgf_product: virial requires forces");
+  if ((hdab || hadb || a_hdab) && compute_tau)
+    fatal("integrate_pgf_product: hdab, hadb and a_hdab are not defined with compute_tau");
+  if ((forces || virials) && !pab) fatal("integrate_pgf_product: forces/virials need pab");
+  ensure_device();
+  PgfContext &c = pgf_ctx();
+  std::lock_guard<std::mutex> g(c.mu);
+  const hipStream_t s = stream();
+  const PgfArgs a{orthorhombic, border_mask, la_max, la_min, lb_max, lb_min, zeta, zetb, 1.0,
+                  dh, dh_inv, ra, rab, npts_global, npts_local, shift_local, border_width,
+                  radius, o1, o2, n1, n2};
+  const size_t npts = (size_t)npts_local[0] * npts_local[1] * npts_local[2];
+  const size_t nab = (size_t)n1 * n2;
+  Level L;
+  L.grid = device_view(grid, npts, c.grid, true, s);
+  const Task *dt;
+  const Group *dg;
+  const Level *dl;
+  const int kind = pgf_records(c, a, L, s, &dt, &dg, &dl);
+  if (kind == kTaskSkip) {
+    HIP_CHECK(hipStreamSynchronize(s));
+    return;  // no point of the grid: hab, forces and derivatives unchanged
+  }
+  int ld[4];
+  integrate_ldiffs(calc_forces, calc_virial, compute_tau, ld);
+  PointParams P{};
+  P.tasks = dt;
+  P.ntasks = 1;
+  P.groups = dg;
+  P.levels = dl;
+  P.cart = calc_forces ? device_view(pab, nab, c.pab, true, s) : nullptr;
+  P.cart_len = (long long)nab;
+  P.hab_cart = device_view(hab, nab, c.hab, true, s);
+  for (int i = 0; i < 4; i++) P.ld[i] = ld[i];
+  P.compute_tau = compute_tau;
+  P.do_forces = calc_forces;
+  P.do_virial = calc_virial;
+  P.fp32 = precision() == GRID_MI355X_FP32_MIXED;
+  P.plan = make_plan(&c.tl, ld, true);
+  double *out = (double *)c.out.get(sizeof(double) * (24 + kTaskOutStride));
+  P.pgf_out = out;
+  P.task_out = out + 24;
+  P.hdab = device_view(hdab, 3 * nab, c.hdab, true, s);
+  P.hadb = device_view(hadb, 3 * nab, c.hadb, true, s);
+  P.a_hdab = device_view(a_hdab, 9 * nab, c.a_hdab, true, s);
+  HIP_CHECK(launch_fill(out, 24 + kTaskOutStride, 0.0, s));
+  HIP_CHECK(launch_integrate(P, kind == kTaskOrtho, s));
+  copy_back(hab, P.hab_cart, nab, s);
+  copy_back(hdab, P.hdab, 3 * nab, s);
+  copy_back(hadb, P.hadb, 3 * nab, s);
+  copy_back(a_hdab, P.a_hdab, 9 * nab, s);
+  double res[24];
+  HIP_CHECK(hipMemcpyAsync(res, out, sizeof res, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (forces) add_into(forces, res, 6, s);
+  if (virials) add_into(virials, res + 6, 18, s);
+}
+
+// ---- buffer packing and residency, src/task_list_methods.F:2561-2674 ----
+void grid_mi355x_pack_buffer(const int npairs, const double *const *blocks,
+                             const int *block_sizes, const int *pair_offsets,
+                             offload_buffer *dest) {
+  if (npairs <= 0) return;
+  if (!dest || !blocks || !block_sizes || !pair_offsets) fatal("pack_buffer: NULL argument");
+  const long long cap = (long long)(dest->size / sizeof(double));
+  std::vector<long long> off(npairs);
+  for (int i = 0; i < npairs; i++) {
+    if (block_sizes[i] < 0 || pair_offsets[i] < 0 ||
+        (long long)pair_offsets[i] + block_sizes[i] > cap)
+      fatal("pack_buffer: pair %d (offset %d, %d doubles) outside the buffer of %lld", i + 1,
+            pair_offsets[i], block_sizes[i], cap);
+    off[i] = pair_offsets[i];
+  }
+  const hipStream_t s = stream();
+  if (!device_pointer(blocks[0]) || !device_pointer(blocks[npairs - 1])) {
+    // host blocks: rs_pack_buffer as the reference does it, into host_buffer
+    for (int i = 0; i < npairs; i++)
+      memcpy(dest->host_buffer + off[i], blocks[i], sizeof(double) * (size_t)block_sizes[i]);
+    set_device_current(dest, false);
+    return;
+  }
+  if (!dest->device_buffer) fatal("pack_buffer: the buffer has no device copy");
+  ensure_device();
+  static DevBuf d_ptr, d_size, d_off;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  upload(d_ptr, std::vector<const double *>(blocks, blocks + npairs));
+  upload(d_size, std::vector<int>(block_sizes, block_sizes + npairs));
+  upload(d_off, off);
+  HIP_CHECK(launch_pack((const double *const *)d_ptr.p, (const int *)d_size.p,
+                        (const long long *)d_off.p, npairs, dest->device_buffer, s));
+  sync_stream(s);
+  set_device_current(dest, true);
+}
+
+void grid_mi355x_unpack_buffer(const int npairs, double *const *blocks, const int *block_sizes,
+                               const int *pair_offsets, const offload_buffer *src) {
+  if (npairs <= 0) return;
+  if (!src || !blocks || !block_sizes || !pair_offsets) fatal("unpack_buffer: NULL argument");
+  const long long cap = (long long)(src->size / sizeof(double));
+  std::vector<long long> off(npairs);
+  for (int i = 0; i < npairs; i++) {
+    if (block_sizes[i] < 0 || pair_offsets[i] < 0 ||
+        (long long)pair_offsets[i] + block_sizes[i] > cap)
+      fatal("unpack_buffer: pair %d (offset %d, %d doubles) outside the buffer of %lld", i + 1,
+            pair_offsets[i], block_sizes[i], cap);
+    off[i] = pair_offsets[i];
+  }
+  const hipStream_t s = stream();
+  const bool dev_current = src->device_buffer && residency_of(src).device_current;
+  if (!device_pointer(blocks[0]) || !device_pointer(blocks[npairs - 1])) {
+    // host blocks: rs_unpack_buffer on the host copy (brought up to date first)
+    if (dev_current) {
+      HIP_CHECK(hipMemcpyAsync(src->host_buffer, src->device_buffer, src->size,
+                               hipMemcpyDeviceToHost, s));
+      sync_stream(s);
+    }
+    for (int i = 0; i < npairs; i++) {
+      const double *p = src->host_buffer + off[i];
+      for (int e = 0; e < block_sizes[i]; e++) blocks[i][e] += p[e];
+    }
+    return;
+  }
+  ensure_device();
+  static DevBuf d_ptr, d_size, d_off, d_src;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  const double *view = dev_current ? src->device_buffer : nullptr;
+  if (!view) {
+    double *d = (double *)d_src.get(src->size ? src->size : sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(d, src->host_buffer, src->size, hipMemcpyHostToDevice, s));
+    view = d;
+  }
+  upload(d_ptr, std::vector<double *>(blocks, blocks + npairs));
+  upload(d_size, std::vector<int>(block_sizes, block_sizes + npairs));
+  upload(d_off, off);
+  HIP_CHECK(launch_unpack((double *const *)d_ptr.p, (const int *)d_size.p,
+                          (const long long *)d_off.p, npairs, view, s));
+  sync_stream(s);
+}
+
+void grid_mi355x_buffer_set_resident(offload_buffer *buffer, const bool resident) {
+  if (!buffer) fatal("buffer_set_resident: NULL buffer");
+  if (resident && !buffer->device_buffer)
+    fatal("buffer_set_resident: the buffer has no device copy (created by another "
+          "offload_create_buffer?)");
+  std::lock_guard<std::mutex> g(res_mu);
+  res_tab[buffer].resident = resident;
+}
+
+void grid_mi355x_buffer_sync_host(offload_buffer *buffer) {
+  if (!buffer) fatal("buffer_sync_host: NULL buffer");
+  if (!buffer->device_buffer || !residency_of(buffer).device_current) return;
+  const hipStream_t s = stream();
+  HIP_CHECK(hipMemcpyAsync(buffer->host_buffer, buffer->device_buffer, buffer->size,
+                           hipMemcpyDeviceToHost, s));
+  sync_stream(s);
+}
+
+void grid_mi355x_buffer_host_written(offload_buffer *buffer) {
+  if (!buffer) fatal("buffer_host_written: NULL buffer");
+  set_device_current(buffer, false);
+}
+
+int grid_mi355x_buffer_device_current(const offload_buffer *buffer) {
+  return buffer && buffer->device_buffer && residency_of(buffer).device_current ? 1 : 0;
 }
 
 void *grid_mi355x_device_malloc(const size_t bytes) {

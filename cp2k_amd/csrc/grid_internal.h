@@ -175,6 +175,12 @@ struct PointParams {
   // gmat + gmat_off[level * kGmatSlots + lp]
   const double *gmat;
   const int *gmat_off;
+  // single-task integration (per-task kernel, one task): if pgf_out is set,
+  // forces a[3], b[3], virials a[9], b[9] go there unscaled, and the hab
+  // derivatives into hdab / hadb ([cart_len][3]) and a_hdab ([cart_len][9])
+  // where set
+  double *pgf_out;
+  double *hdab, *hadb, *a_hdab;
 };
 
 // Task of position t of a launch.

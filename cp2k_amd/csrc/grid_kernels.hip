@@ -686,40 +686,61 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
   const double za = T.zeta, zb = T.zetb;
   const int a0 = ncoset(T.la_min - 1), na = ncoset(T.la_max) - a0;
   const int b0 = ncoset(T.lb_min - 1), nb = ncoset(T.lb_max) - b0;
+  // Single-task mode (P.pgf_out set, grid_ref_integrate.c:95-203): forces of
+  // a and b, virials of a and b apart, and the unweighted derivatives hdab,
+  // hadb, a_hdab of each hab element ([n2][n1][3] and [n2][n1][3][3] with
+  // the element's Cartesian index q; tau is off for those).
+  const bool pgf = P.pgf_out != nullptr;
   double f[6] = {0, 0, 0, 0, 0, 0};
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double va[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double vb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int e = tid; e < na * nb; e += nthr) {
     const int ia = a0 + e % na, jb = b0 + e / na;
     const Ang A = orb(ia), B = orb(jb);
     const long long q = G.cart_off + (long long)(T.o2 + jb) * G.ncoa + T.o1 + ia;
     if (q < P.cart_len) atomicAdd(&P.hab_cart[q], get_hab(A, B, za, zb, cg, tau));
     if (FULL && P.do_forces) {
-      const double pv = (q < P.cart_len) ? P.cart[q] : 0.0;
+      const double pv = (P.cart && q < P.cart_len) ? P.cart[q] : 0.0;
       for (int i = 0; i < 3; i++) {
-        f[i] += pv * get_fa(A, B, i, za, zb, cg, tau);
-        f[3 + i] += pv * get_fb(A, B, i, za, zb, T.rab, cg, tau);
+        const double fa = get_fa(A, B, i, za, zb, cg, tau);
+        const double fb = get_fb(A, B, i, za, zb, T.rab, cg, tau);
+        f[i] += pv * fa;
+        f[3 + i] += pv * fb;
+        if (pgf && P.hdab && q < P.cart_len) P.hdab[3 * q + i] += fa;
+        if (pgf && P.hadb && q < P.cart_len) P.hadb[3 * q + i] += fb;
       }
       if (P.do_virial) {
         for (int i = 0; i < 3; i++)
           for (int j = 0; j < 3; j++) {
-            v[i * 3 + j] += pv * get_va(A, B, i, j, za, zb, cg, tau);
-            v[i * 3 + j] += pv * get_vb(A, B, i, j, za, zb, T.rab, cg, tau);
+            const double a = get_va(A, B, i, j, za, zb, cg, tau);
+            va[i * 3 + j] += pv * a;
+            vb[i * 3 + j] += pv * get_vb(A, B, i, j, za, zb, T.rab, cg, tau);
+            if (pgf && P.a_hdab && q < P.cart_len) P.a_hdab[9 * q + 3 * i + j] += a;
           }
       }
     }
   }
   if (FULL && P.do_forces) {
     const int lane = tid & 63, w = tid >> 6;
-    for (int c = 0; c < kTaskOutStride; c++) {
-      const double x = wave_sum(c < 6 ? f[c] : v[c - 6]);
-      if (lane == 0) red[w * kTaskOutStride + c] = x;
+    // list mode: forces and the summed virial (15); single task: 6 + 9 + 9
+    const int nout = pgf ? 24 : kTaskOutStride;
+    for (int c = 0; c < nout; c++) {
+      const double x = c < 6 ? f[c]
+                     : pgf   ? (c < 15 ? va[c - 6] : vb[c - 15])
+                             : va[c - 6] + vb[c - 6];
+      const double sum = wave_sum(x);
+      if (lane == 0) red[w * nout + c] = sum;
     }
     __syncthreads();
-    if (tid < kTaskOutStride) {
+    if (tid < nout) {
       double s = 0.0;
-      for (int q = 0; q < NW; q++) s += red[q * kTaskOutStride + tid];
-      const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
-      out[tid] = scale * s;
+      for (int q = 0; q < NW; q++) s += red[q * nout + tid];
+      if (pgf) {
+        P.pgf_out[tid] = s;
+      } else {
+        const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
+        out[tid] = scale * s;
+      }
     }
   }
 }

@@ -38,6 +38,10 @@ EXPORTS = [
     "grid_mi355x_batch_grid", "grid_mi355x_batch_run", "grid_mi355x_batch_task_points",
     "grid_mi355x_autotune", "grid_mi355x_tuned_shape", "grid_mi355x_point_launches",
     "grid_mi355x_collocate_task_list_noalloc",
+    "grid_mi355x_collocate_pgf_product", "grid_mi355x_integrate_pgf_product",
+    "grid_mi355x_pack_buffer", "grid_mi355x_unpack_buffer", "grid_mi355x_buffer_set_resident",
+    "grid_mi355x_buffer_sync_host", "grid_mi355x_buffer_host_written",
+    "grid_mi355x_buffer_device_current",
 ]
 
 GRID_FUNC_AB = 100
@@ -117,6 +121,16 @@ def lib():
         L.grid_mi355x_device_malloc.argtypes = [C.c_size_t]
         L.grid_mi355x_device_free.argtypes = [C.c_void_p]
         L.grid_mi355x_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.grid_mi355x_collocate_pgf_product.restype = None
+        L.grid_mi355x_pack_buffer.argtypes = [C.c_int, C.POINTER(C.c_void_p), _ip, _ip,
+                                              C.POINTER(OffloadBuffer)]
+        L.grid_mi355x_unpack_buffer.argtypes = [C.c_int, C.POINTER(C.c_void_p), _ip, _ip,
+                                                C.POINTER(OffloadBuffer)]
+        L.grid_mi355x_buffer_set_resident.argtypes = [C.POINTER(OffloadBuffer), C.c_bool]
+        L.grid_mi355x_buffer_sync_host.argtypes = [C.POINTER(OffloadBuffer)]
+        L.grid_mi355x_buffer_host_written.argtypes = [C.POINTER(OffloadBuffer)]
+        L.grid_mi355x_buffer_device_current.argtypes = [C.POINTER(OffloadBuffer)]
+        L.grid_mi355x_integrate_pgf_product.restype = None
         _lib = L
     return _lib
 
@@ -161,6 +175,19 @@ class OffloadBuf:
         if self.length == 0:
             return np.zeros(0)
         return np.ctypeslib.as_array(self.h.contents.host_buffer, shape=(self.length,))
+
+    # ---- residency (include/grid_buffers.h) ----
+    def set_resident(self, resident=True):
+        lib().grid_mi355x_buffer_set_resident(self.h, C.c_bool(resident))
+
+    def sync_host(self):
+        lib().grid_mi355x_buffer_sync_host(self.h)
+
+    def host_written(self):
+        lib().grid_mi355x_buffer_host_written(self.h)
+
+    def device_current(self):
+        return bool(lib().grid_mi355x_buffer_device_current(self.h))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -384,6 +411,104 @@ class AbiBuffers:
     def free(self):
         for b in [self.pab, self.hab] + self.rho + self.pot:
             b.__del__()
+
+
+# ---- matrix blocks <-> buffers (include/grid_buffers.h) ----
+def _block_args(blocks, pair_offsets):
+    ptrs = (C.c_void_p * len(blocks))(*[
+        b.ptr if isinstance(b, DeviceArray) else b.ctypes.data for b in blocks])
+    sizes = np.array([b.size for b in blocks], dtype=np.int32)
+    offs = np.ascontiguousarray(pair_offsets, dtype=np.int32)
+    return ptrs, sizes, offs
+
+
+def pack_buffer(blocks, pair_offsets, dest):
+    """grid_mi355x_pack_buffer: blocks (all DeviceArray or all float64 numpy
+    arrays, column-major data) into OffloadBuf dest at pair_offsets."""
+    ptrs, sizes, offs = _block_args(blocks, pair_offsets)
+    lib().grid_mi355x_pack_buffer(C.c_int(len(blocks)), ptrs, _ptr(sizes, C.c_int),
+                                  _ptr(offs, C.c_int), dest.h)
+
+
+def unpack_buffer(blocks, pair_offsets, src):
+    """grid_mi355x_unpack_buffer: blocks[i] += src at pair_offsets[i]."""
+    ptrs, sizes, offs = _block_args(blocks, pair_offsets)
+    lib().grid_mi355x_unpack_buffer(C.c_int(len(blocks)), ptrs, _ptr(sizes, C.c_int),
+                                    _ptr(offs, C.c_int), src.h)
+
+
+# ---- single task (include/grid_pgf_product.h), shaped like the Fortran
+#      collocate_pgf_product / integrate_pgf_product, src/grid/grid_api.F:110-490 ----
+def _d64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _i32(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+def _pgf_geometry(t):
+    return [_d64(t.dh), _d64(t.dh_inv), _d64(t.ra), _d64(t.rab), _i32(t.npts_global),
+            _i32(t.npts_local), _i32(t.shift_local), _i32(t.border_width)]
+
+
+def _arg(x, ctype=C.c_double):
+    """A host numpy array or a DeviceArray (device memory) as a pointer."""
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return C.cast(C.c_void_p(x.ptr), C.POINTER(ctype))
+    return _ptr(x, ctype)
+
+
+def collocate_pgf(t, pab=None, func=None, rscale=None, grid=None):
+    """grid_mi355x_collocate_pgf_product of task t (attributes as the
+    reference's .task files: l ranges, exponents, geometry, o1/o2/n1/n2).
+    grid: host array [nz, ny, nx] or DeviceArray, accumulated into (a new
+    zero host grid if None); returns it."""
+    pab = _d64(t.pab if pab is None else pab) if not isinstance(pab, DeviceArray) else pab
+    nx, ny, nz = [int(v) for v in t.npts_local]
+    if grid is None:
+        grid = np.zeros((nz, ny, nx))
+    g = _pgf_geometry(t)
+    lib().grid_mi355x_collocate_pgf_product(
+        C.c_bool(bool(t.orthorhombic)), C.c_int(int(t.border_mask)),
+        C.c_int(int(t.func if func is None else func)), C.c_int(int(t.la_max)),
+        C.c_int(int(t.la_min)), C.c_int(int(t.lb_max)), C.c_int(int(t.lb_min)),
+        C.c_double(t.zeta), C.c_double(t.zetb),
+        C.c_double(t.rscale if rscale is None else rscale), _ptr(g[0]), _ptr(g[1]),
+        _ptr(g[2]), _ptr(g[3]), *[_ptr(x, C.c_int) for x in g[4:]], C.c_double(t.radius),
+        C.c_int(int(t.o1)), C.c_int(int(t.o2)), C.c_int(int(t.n1)), C.c_int(int(t.n2)),
+        _arg(pab), _arg(grid))
+    return grid
+
+
+def integrate_pgf(t, grid=None, compute_tau=False, forces=True, virial=True,
+                  derivs=False, hab=None):
+    """grid_mi355x_integrate_pgf_product of task t: returns (hab [n2, n1],
+    forces [2, 3] or None, virials [2, 3, 3] or None) and with derivs also
+    (hdab, hadb, a_hdab).  grid: host array or DeviceArray (t.grid if None);
+    hab: host array or DeviceArray accumulated into (zeros if None)."""
+    grid = _d64(t.grid if grid is None else grid) if not isinstance(grid, DeviceArray) else grid
+    if hab is None:
+        hab = np.zeros((int(t.n2), int(t.n1)))
+    f = np.zeros((2, 3)) if forces else None
+    v = np.zeros((2, 3, 3)) if virial else None
+    n2, n1 = int(t.n2), int(t.n1)
+    d3 = [np.zeros((n2, n1, 3)), np.zeros((n2, n1, 3)),
+          np.zeros((n2, n1, 3, 3))] if derivs else [None, None, None]
+    g = _pgf_geometry(t)
+    lib().grid_mi355x_integrate_pgf_product(
+        C.c_bool(bool(t.orthorhombic)), C.c_bool(bool(compute_tau)),
+        C.c_int(int(t.border_mask)), C.c_int(int(t.la_max)), C.c_int(int(t.la_min)),
+        C.c_int(int(t.lb_max)), C.c_int(int(t.lb_min)), C.c_double(t.zeta),
+        C.c_double(t.zetb), _ptr(g[0]), _ptr(g[1]), _ptr(g[2]), _ptr(g[3]),
+        *[_ptr(x, C.c_int) for x in g[4:]], C.c_double(t.radius), C.c_int(int(t.o1)),
+        C.c_int(int(t.o2)), C.c_int(n1), C.c_int(n2), _arg(grid), _arg(hab),
+        _arg(_d64(t.pab)), _ptr(f), _ptr(v), *[_ptr(x) for x in d3])
+    if derivs:
+        return hab, f, v, d3
+    return hab, f, v
 
 
 def set_stream(stream_handle):

@@ -809,9 +809,27 @@ void oracle_integrate_pgf_product(
     const int shift_local[3], const int border_width[3], double radius, int o1,
     int o2, int n1, int n2, const double *grid, double *hab, const double *pab,
     double *forces, double *virials) {
+  oracle_integrate_pgf_product_full(
+      orthorhombic, compute_tau, border_mask, la_max, la_min, lb_max, lb_min, zeta, zetb, dh,
+      dh_inv, ra, rab, npts_global, npts_local, shift_local, border_width, radius, o1, o2, n1,
+      n2, grid, hab, pab, forces, virials, NULL, NULL, NULL);
+}
+
+void oracle_integrate_pgf_product_full(
+    bool orthorhombic, bool compute_tau, int border_mask, int la_max,
+    int la_min, int lb_max, int lb_min, double zeta, double zetb,
+    const double dh[3][3], const double dh_inv[3][3], const double ra[3],
+    const double rab[3], const int npts_global[3], const int npts_local[3],
+    const int shift_local[3], const int border_width[3], double radius, int o1,
+    int o2, int n1, int n2, const double *grid, double *hab, const double *pab,
+    double *forces, double *virials, double *hdab, double *hadb, double *a_hdab) {
   (void)n2;
-  const bool do_f = forces != NULL, do_v = virials != NULL;
+  /* grid_ref_integrate.c:56-60: the derivative outputs widen l like forces
+   * (hdab, hadb) and the virial (a_hdab) */
+  const bool do_f = forces != NULL || hdab != NULL || hadb != NULL;
+  const bool do_v = virials != NULL || a_hdab != NULL;
   assert(!do_v || do_f);
+  assert(!(hdab || hadb || a_hdab) || !compute_tau);
   int dla_max = 0, dla_min = 0, dlb_max = 0, dlb_min = 0;
   if (do_f || do_v) { dla_max += 1; dla_min -= 1; dlb_min -= 1; }
   if (do_v) { dla_max += 1; dlb_max += 1; }
@@ -836,14 +854,14 @@ void oracle_integrate_pgf_product(
               const ang b = {{bx, by, lb - bx - by}};
               const long e = (long)(o2 + aidx(b)) * n1 + o1 + aidx(a);
               hab[e] += get_hab(a, b, za, zb, &c, compute_tau);
-              if (do_f) {
+              if (forces) {
                 const double pv = pab[e];
                 for (int i = 0; i < 3; i++) {
                   forces[i] += pv * get_fa(a, b, i, za, zb, &c, compute_tau);
                   forces[3 + i] += pv * get_fb(a, b, i, za, zb, rab, &c, compute_tau);
                 }
               }
-              if (do_v) {
+              if (virials) {
                 const double pv = pab[e];
                 for (int i = 0; i < 3; i++)
                   for (int j = 0; j < 3; j++) {
@@ -851,6 +869,15 @@ void oracle_integrate_pgf_product(
                     virials[9 + i * 3 + j] +=
                         pv * get_vb(a, b, i, j, za, zb, rab, &c, compute_tau);
                   }
+              }
+              /* grid_ref_integrate.c:126-149: the same getters without the
+               * density weight, per hab element */
+              for (int i = 0; i < 3; i++) {
+                if (hdab) hdab[3 * e + i] += get_fa(a, b, i, za, zb, &c, false);
+                if (hadb) hadb[3 * e + i] += get_fb(a, b, i, za, zb, rab, &c, false);
+                if (a_hdab)
+                  for (int j = 0; j < 3; j++)
+                    a_hdab[9 * e + 3 * i + j] += get_va(a, b, i, j, za, zb, &c, false);
               }
             }
       }

@@ -72,6 +72,20 @@ void oracle_integrate_pgf_product(
     int o2, int n1, int n2, const double *grid, double *hab, const double *pab,
     double *forces, double *virials);
 
+/* The same with the reference's full argument list
+ * (src/grid/ref/grid_ref_integrate.c:44-154): hdab, hadb ([n2][n1][3]) and
+ * a_hdab ([n2][n1][3][3]) are the unweighted force / virial getters per hab
+ * element, optional (NULL), accumulated into, not defined with compute_tau. */
+void oracle_integrate_pgf_product_full(
+    bool orthorhombic, bool compute_tau, int border_mask, int la_max,
+    int la_min, int lb_max, int lb_min, double zeta, double zetb,
+    const double dh[3][3], const double dh_inv[3][3], const double ra[3],
+    const double rab[3], const int npts_global[3], const int npts_local[3],
+    const int shift_local[3], const int border_width[3], double radius, int o1,
+    int o2, int n1, int n2, const double *grid, double *hab, const double *pab,
+    double *forces, double *virials, double *hdab, double *hadb,
+    double *a_hdab);
+
 /* Number of (task, grid point) pairs the single-task kernels visit. */
 long oracle_task_points(bool orthorhombic, int border_mask, int lp,
                         double zeta, double zetb, const double dh[3][3],

@@ -76,6 +76,7 @@ def lib():
         L = C.CDLL(str(LIB))
         L.oracle_collocate_pgf_product.restype = None
         L.oracle_integrate_pgf_product.restype = None
+        L.oracle_integrate_pgf_product_full.restype = None
         L.oracle_task_points.restype = C.c_long
         L.oracle_create_task_list.restype = C.c_void_p
         L.oracle_free_task_list.argtypes = [C.c_void_p]
@@ -123,14 +124,19 @@ def collocate_pgf(t, pab=None, func=None, rscale=None, grid=None, libh=None,
 
 
 def integrate_pgf(t, grid=None, compute_tau=None, forces=True, virial=True,
-                  libh=None, prefix="oracle"):
-    """Single-task integration; returns (hab, forces[2,3], virials[2,3,3])."""
+                  libh=None, prefix="oracle", derivs=False):
+    """Single-task integration; returns (hab, forces[2,3], virials[2,3,3]),
+    with derivs also (hdab [n2,n1,3], hadb [n2,n1,3], a_hdab [n2,n1,3,3])
+    (the oracle's full form or the reference's own signature)."""
     L = libh or lib()
-    fn = getattr(L, f"{prefix}_integrate_pgf_product")
+    full = derivs and prefix == "oracle"
+    fn = getattr(L, f"{prefix}_integrate_pgf_product" + ("_full" if full else ""))
     grid = _d(t.grid if grid is None else grid)
     hab = np.zeros((t.n2, t.n1))
     f = np.zeros((2, 3)) if forces else None
     v = np.zeros((2, 3, 3)) if virial else None
+    d3 = [np.zeros((t.n2, t.n1, 3)), np.zeros((t.n2, t.n1, 3)),
+          np.zeros((t.n2, t.n1, 3, 3))] if derivs else [None, None, None]
     pab = _d(t.pab)
     dh, dh_inv, ra, rab, ng, nl, sh, bw = _single_args(t)
     tau = t.compute_tau if compute_tau is None else compute_tau
@@ -142,9 +148,11 @@ def integrate_pgf(t, grid=None, compute_tau=None, forces=True, virial=True,
             C.c_double(t.radius), C.c_int(t.o1), C.c_int(t.o2),
             C.c_int(t.n1), C.c_int(t.n2), _ptr(grid), _ptr(hab), _ptr(pab),
             _ptr(f), _ptr(v)]
-    if prefix != "oracle":  # reference signature adds hdab, hadb, a_hdab
-        args += [None, None, None]
+    if prefix != "oracle" or full:  # reference signature adds hdab, hadb, a_hdab
+        args += [_ptr(x) for x in d3]
     fn(*args)
+    if derivs:
+        return hab, f, v, d3
     return hab, f, v
 
 

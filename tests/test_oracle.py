@@ -135,6 +135,85 @@ def test_oracle_vs_reference_kernels(seed):
             assert replay.rel_err(v1, v2) < 1e-10
 
 
+@needs_ref
+@pytest.mark.parametrize("ortho", [True, False])
+def test_oracle_hab_derivatives_vs_reference(ortho):
+    """hdab / hadb / a_hdab (grid_ref_integrate.c:126-149) against the
+    reference's own single-task integrate, with and without forces."""
+    rng = np.random.default_rng(300 + ortho)
+    for rep in range(4):
+        t = _random_task(rng, 100, ortho, border=(rep == 3 and ortho))
+        V = rng.normal(size=tuple(int(x) for x in t.npts_local[::-1]))
+        with_f = rep % 2 == 0
+        a = O.integrate_pgf(t, grid=V, compute_tau=False, forces=with_f, virial=with_f,
+                            derivs=True)
+        b = O.integrate_pgf(t, grid=V, compute_tau=False, forces=with_f, virial=with_f,
+                            libh=ref, prefix="grid_ref", derivs=True)
+        assert replay.rel_err(a[0], b[0]) < TOL
+        if with_f:
+            assert replay.rel_err(a[1], b[1]) < 1e-10
+            assert replay.rel_err(a[2], b[2]) < 1e-10
+        for x, y in zip(a[3], b[3]):
+            assert replay.rel_err(x, y) < 1e-10
+
+
+@pytest.mark.parametrize("ortho", [True, False])
+def test_hab_derivatives_finite_difference(ortho):
+    """hdab = d hab / d ra and hadb = d hab / d rb (the other centre fixed),
+    by central differences of the oracle's hab with truncation invisible;
+    the density-weighted sums of hdab / hadb are the forces on a / b."""
+    rng = np.random.default_rng(41 + ortho)
+    t = brute.SyntheticTask(rng, (0, 2), (1, 2), ortho=ortho)
+    V = rng.normal(size=tuple(int(x) for x in t.npts_local[::-1]))
+    hab, f, v, (hdab, hadb, a_hdab) = O.integrate_pgf(t, grid=V, derivs=True)
+    d = 1e-5
+    ra0, rab0 = t.ra.copy(), t.rab.copy()
+    for i in range(3):
+        e = np.zeros(3)
+        e[i] = d
+        hp = []
+        for sgn in (1, -1):
+            t.ra, t.rab = ra0 + sgn * e, rab0 - sgn * e  # move a, b stays
+            hp.append(O.integrate_pgf(t, grid=V, forces=False, virial=False)[0])
+        fd_a = (hp[0] - hp[1]) / (2 * d)
+        hp = []
+        for sgn in (1, -1):
+            t.ra, t.rab = ra0, rab0 + sgn * e  # move b, a stays
+            hp.append(O.integrate_pgf(t, grid=V, forces=False, virial=False)[0])
+        fd_b = (hp[0] - hp[1]) / (2 * d)
+        t.ra, t.rab = ra0, rab0
+        scale = np.max(np.abs(fd_a)) + np.max(np.abs(fd_b))
+        assert np.max(np.abs(hdab[..., i] - fd_a)) < 1e-6 * scale
+        assert np.max(np.abs(hadb[..., i] - fd_b)) < 1e-6 * scale
+        mask = np.zeros_like(t.pab)
+        a0 = brute._ncoset(t.la_min - 1)
+        b0 = brute._ncoset(t.lb_min - 1)
+        mask[t.o2 + b0:t.o2 + brute._ncoset(t.lb_max), t.o1 + a0:t.o1 + brute._ncoset(t.la_max)] = 1
+        pm = t.pab * mask
+        assert abs(np.sum(pm * hdab[..., i]) - f[0, i]) < 1e-10 * np.max(np.abs(f))
+        assert abs(np.sum(pm * hadb[..., i]) - f[1, i]) < 1e-10 * np.max(np.abs(f))
+    assert np.allclose(np.einsum("ba,baij->ij", pm, a_hdab), v[0], rtol=0, atol=1e-10 * np.max(np.abs(v)))
+
+
+def test_buffer_pack_unpack():
+    """rs_pack_buffer / rs_unpack_buffer restatement: the blocks of a
+    generated density matrix pack back into the generator's buffer, and
+    pack -> unpack into zero blocks returns the blocks (any order)."""
+    from oracle import buffers as B
+    args, pab, _ = synth.config("h2o2", seed=4, n=16)
+    offs = np.asarray(args.block_offsets)
+    sizes = np.diff(np.append(offs, args.buffer_size))
+    blocks = [pab[o:o + n].copy() for o, n in zip(offs, sizes)]
+    perm = np.random.default_rng(2).permutation(len(blocks))
+    buf = B.pack_buffer([blocks[i] for i in perm], offs[perm], args.buffer_size)
+    assert np.array_equal(buf, pab)
+    back = [np.zeros(n) for n in sizes]
+    B.unpack_buffer(back, offs, buf)
+    assert all(np.array_equal(a, b) for a, b in zip(back, blocks))
+    B.unpack_buffer(back, offs, buf)
+    assert all(np.array_equal(a, 2 * b) for a, b in zip(back, blocks))
+
+
 # ---- task-list layer ----
 def _tasks_by_hand(args, pab_blocks, func):
     """Collocation of a task list via numpy decontraction + single-task oracle."""
