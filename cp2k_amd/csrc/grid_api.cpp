@@ -1576,8 +1576,9 @@ WavePlan general_col_plan(const TaskListImpl *tl, int level, int lp) {
   w.o_q = 8 * nd;
   w.q_cap = ni + nj + nk;
   w.o_rows = w.o_q + 8 * w.q_cap;
-  // a slab holds at least one plane; 4096 rows (16 KiB) cover most tasks whole
-  w.rows_cap = (int)std::max<long long>(nj, std::min<long long>(rows, 4096));
+  // a slab holds at least one plane; 1024 rows (4 KiB) keep the wave region
+  // small (more waves per CU) and cover most tasks in one or two slabs
+  w.rows_cap = (int)std::max<long long>(nj, std::min<long long>(rows, 1024));
   w.o_jinfo = (w.o_rows + 4 * w.rows_cap + 15) & ~15;
   w.j_cap = nj;
   w.o_cstart = w.o_jinfo + 16 * nj;

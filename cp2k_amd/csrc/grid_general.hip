@@ -291,7 +291,8 @@ __device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, 
 // per column the moments times di^il dj^jl into the task's cijk sums
 // (lane-private registers for lp <= 6, the matrix cores for lp 7..11).
 // ---------------------------------------------------------------------------
-constexpr int kColRestart = 64;  // planes between exact exp() evaluations of g
+constexpr int kColRestart = 64;  // planes between exact exp() evaluations of g (FP64)
+constexpr int kColBatch = 8;     // potential loads in flight per lane
 constexpr int kColStage = 40;    // MFMA staging per column: di^il[16], dj^jl[12], M[12]
 
 struct ColLds {
@@ -471,27 +472,25 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
       if (klo <= khi) {
         const double qij = W.q[x] * W.q[ni + jj];
         const int cb = gy * nx + gx;
+        // the Gaussian recurrence stays FP64 in both modes (in FP32 its
+        // relative drift grows with the square of the planes walked)
         for (int k0 = klo; k0 <= khi; k0 += kColRestart) {
           const int k1 = min(khi, k0 + kColRestart - 1);
           const int kabs = T.lo[2] + kb + k0;
-          double dk = kabs - T.gp[2];
+          const double dk0 = kabs - T.gp[2];
           double r2 = 0.0;
 #pragma unroll
           for (int d = 0; d < 3; d++) {
-            const double rd = di * h0[d] + dj * h1[d] + dk * h2[d];
+            const double rd = di * h0[d] + dj * h1[d] + dk0 * h2[d];
             r2 += rd * rd;
           }
           double g = exp(-T.zetp * r2);
           double q = qij * W.q[ni + nj + kb + k0];
+          double dk = dk0;
           int gz = wrap(kabs - lay.shift_local[2], ngz);
           const int *rowk = W.rows + k0 * nj + jj;
-#pragma unroll 2
-          for (int kk = k0; kk <= k1; kk++, rowk += nj) {
-            const int v = *rowk;
-            const bool in = act && (unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16);
-            R val = R(0);
-            if (in) val = grid[gz * nxy + cb];
-            R t = (R)((double)val * g);
+          auto step = [&](R val) {
+            R t = val * (R)g;
             const R dkr = (R)dk;
 #pragma unroll
             for (int l = 0; l <= LP; l++) {
@@ -501,6 +500,29 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
             g *= q;
             q *= qq;
             dk += 1.0;
+          };
+          int kk = k0;
+          // kColBatch planes at a time: their loads are unconditional (index
+          // 0 stands in for a point outside the row) and in flight together
+          for (; kk + kColBatch - 1 <= k1; kk += kColBatch) {
+            R val[kColBatch];
+#pragma unroll
+            for (int u = 0; u < kColBatch; u++) {
+              const int v = rowk[u * nj];
+              const bool in = act && (unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16);
+              const R w = grid[in ? gz * nxy + cb : 0];
+              val[u] = in ? w : R(0);
+              gz = (gz + 1 == ngz) ? 0 : gz + 1;
+            }
+            rowk += kColBatch * nj;
+#pragma unroll
+            for (int u = 0; u < kColBatch; u++) step(val[u]);
+          }
+          for (; kk <= k1; kk++, rowk += nj) {
+            const int v = *rowk;
+            const bool in = act && (unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16);
+            const R w = grid[in ? gz * nxy + cb : 0];
+            step(in ? w : R(0));
             gz = (gz + 1 == ngz) ? 0 : gz + 1;
           }
         }
