@@ -1512,10 +1512,14 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         // task per lane scatters them over 64 blocks and measured slower);
         // lp 7..11: a workgroup per task (the tables fill most of LDS)
         Q.lane_cxyz = 0;
-        if (c2 > o.begin) {
+        for (int v = sh; v <= 6; v++) {  // one launch per lp (LDS plan of that lp)
+          const int a = std::max(o.begin, first(v - sh));
+          const int b = std::min(c2, v == 6 ? c2 : first(v + 1 - sh));
+          if (b <= a) continue;
           PointParams F = Q;
-          F.ntasks = c2 - o.begin;
-          F.wp = make_wave_plan(tl, l, std::min(6, tl->lp0s[c2 - 1] + sh), P.ld, kPlanFinish);
+          F.task_base = a;
+          F.ntasks = b - a;
+          F.wp = make_wave_plan(tl, l, v, P.ld, kPlanFinish);
           HIP_CHECK(launch_finish_wave(F, s));
         }
         if (c3 > c2) {
@@ -1706,10 +1710,20 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         HIP_CHECK(launch_general_collocate(R, lpb, s));
       }
       if (mode == kIntegrate) {
+        // one launch per lp, each with the LDS plan of that lp (the small-l
+        // tasks, most of them, then run many more waves per CU)
         TimedLaunch tm(10, s);
         Q.lane_cxyz = 0;
-        Q.wp = make_wave_plan(tl, l, lp_all, P.ld, kPlanFinish);
-        HIP_CHECK(launch_finish_wave(Q, s));
+        for (int v = sh; v <= 6; v++) {
+          const int a = std::max(g.begin, first(v - sh));
+          const int b = std::min(c2, v == 6 ? c2 : first(v + 1 - sh));
+          if (b <= a) continue;
+          PointParams F = Q;
+          F.task_base = a;
+          F.ntasks = b - a;
+          F.wp = make_wave_plan(tl, l, v, P.ld, kPlanFinish);
+          HIP_CHECK(launch_finish_wave(F, s));
+        }
       }
     }
     if (c3 > c2) {  // integrate, lp 7..11: the column sums on the matrix cores

@@ -495,7 +495,9 @@ __device__ __forceinline__ void task_sync() {
     __syncthreads();
 }
 
-template <bool FULL, int NT>
+// KIND: 0 hab only, 1 hab with forces / virial (no tau: the plain getters
+// only, which keeps the register count low), 2 anything (tau getters).
+template <int KIND, int NT>
 __device__ void task_finish(const PointParams &P, const Task &T, const Group &G, int la, int lb,
                             int la_min, int lb_min, const WaveLds &W, double *red,
                             long long task, int tid) {
@@ -548,7 +550,8 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
   }
   task_sync<NT>();
   const CabGet cg{cab, n1};
-  const bool tau = FULL && P.compute_tau != 0;
+  constexpr bool FULL = KIND > 0;
+  const bool tau = KIND == 2 && P.compute_tau != 0;
   const double za = T.zeta, zb = T.zetb;
   const int a0 = ncoset(T.la_min - 1), na = ncoset(T.la_max) - a0;
   const int b0 = ncoset(T.lb_min - 1), nb = ncoset(T.lb_max) - b0;
@@ -562,7 +565,7 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
     if (FULL && P.do_forces && !tau) {
       const double pv = (q < P.cart_len) ? P.cart[q] : 0.0;
       pair_forces_plain(A, B, za, zb, T.rab, cg, pv, P.do_virial != 0, f, v);
-    } else if (FULL && P.do_forces) {
+    } else if (KIND == 2 && P.do_forces) {
       const double pv = (q < P.cart_len) ? P.cart[q] : 0.0;
       for (int i = 0; i < 3; i++) {
         f[i] += pv * get_fa(A, B, i, za, zb, cg, tau);
@@ -1034,8 +1037,8 @@ __global__ __launch_bounds__(256) void integrate_mfma_kernel(const Task *__restr
 }
 
 // Packed cxyz of one task per wave -> hab, forces, virial.
-template <bool FULL>
-__global__ __launch_bounds__(256, (FULL ? 2 : 4)) void finish_wave_kernel(
+template <int KIND>
+__global__ __launch_bounds__(256, (KIND == 2 ? 2 : (KIND == 1 ? 3 : 4))) void finish_wave_kernel(
     const Task *__restrict__ tasks, const Group *__restrict__ groups, PointParams P) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x & 63;
@@ -1050,13 +1053,13 @@ __global__ __launch_bounds__(256, (FULL ? 2 : 4)) void finish_wave_kernel(
   const int nc = (l.lp + 1) * (l.lp + 2) * (l.lp + 3) / 6;
   for (int e = lane; e < nc; e += 64) W.cxyz[e] = in[e];  // packed
   wave_sync();
-  task_finish<FULL, 64>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, W, nullptr, task,
+  task_finish<KIND, 64>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, W, nullptr, task,
                         lane);
 }
 
 // The same per task with a whole workgroup (256 threads) on one task: for
 // high l, where one task's tables fill most of LDS.
-template <bool FULL>
+template <int KIND>
 __global__ __launch_bounds__(256) void finish_block_kernel(const Task *__restrict__ tasks,
                                                            const Group *__restrict__ groups,
                                                            PointParams P) {
@@ -1070,7 +1073,7 @@ __global__ __launch_bounds__(256) void finish_block_kernel(const Task *__restric
   const int nc = (l.lp + 1) * (l.lp + 2) * (l.lp + 3) / 6;
   for (int e = threadIdx.x; e < nc; e += 256) W.cxyz[e] = in[e];  // packed
   __syncthreads();
-  task_finish<FULL, 256>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, W, red, task,
+  task_finish<KIND, 256>(P, T, groups[T.group], l.la, l.lb, l.la_min, l.lb_min, W, red, task,
                          threadIdx.x);
 }
 
@@ -1178,15 +1181,16 @@ hipError_t launch_finish_block(const PointParams &P, hipStream_t s) {
     hipLaunchKernelGGL(kernel, dim3(P.ntasks), dim3(256), lds, s, P.tasks, P.groups, P);
     return hipGetLastError();
   };
-  return full ? go(finish_block_kernel<true>) : go(finish_block_kernel<false>);
+  if (P.compute_tau) return go(finish_block_kernel<2>);
+  return full ? go(finish_block_kernel<1>) : go(finish_block_kernel<0>);
 }
 
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   if (P.wp.nwaves < 1 || P.wp.nwaves > 4 || P.wp.window != 0) return hipErrorInvalidValue;
-  const bool full = P.do_forces || P.compute_tau;
-  return full ? launch_wave(finish_wave_kernel<true>, P, wave_blocks(P), s, P.tasks, P.groups)
-              : launch_wave(finish_wave_kernel<false>, P, wave_blocks(P), s, P.tasks, P.groups);
+  if (P.compute_tau) return launch_wave(finish_wave_kernel<2>, P, wave_blocks(P), s, P.tasks, P.groups);
+  return P.do_forces ? launch_wave(finish_wave_kernel<1>, P, wave_blocks(P), s, P.tasks, P.groups)
+                     : launch_wave(finish_wave_kernel<0>, P, wave_blocks(P), s, P.tasks, P.groups);
 }
 
 }  // namespace gm

@@ -51,11 +51,15 @@ def test_pack_collocate_resident(name):
         r.set_resident(True)
     abi.collocate(100)
     want = O.TaskList(args).collocate(100, B.pack_buffer(blocks, offs, args.buffer_size))
+    # tri4's coarsest level is 8^3 points: thousands of periodic images add
+    # into each point, so FP64 summation-order noise reaches ~1.5e-12 there
+    # (the wrap-stress bar of the parity tests)
+    tol = 1e-11 if name == "tri4" else TOL
     for r, w in zip(abi.rho, want):
         assert r.device_current()
         assert np.isnan(r.host()).all()  # results stayed in HBM
         r.sync_host()
-        assert replay.rel_err(r.host().reshape(w.shape), w) < TOL
+        assert replay.rel_err(r.host().reshape(w.shape), w) < tol
     # the same blocks packed from host memory give the same buffer
     hostbuf = grid.OffloadBuf(args.buffer_size)
     grid.pack_buffer(blocks, offs, hostbuf)
