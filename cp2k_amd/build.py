@@ -17,7 +17,8 @@ LIB = PKG / "libcp2k_grid_mi355x.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = [CSRC / "grid_api.cpp", CSRC / "grid_kernels.hip", CSRC / "grid_wave.hip",
-           CSRC / "grid_general.hip", CSRC / "grid_noalloc.hip", CSRC / "grid_buffers.hip"]
+           CSRC / "grid_general.hip", CSRC / "grid_noalloc.hip", CSRC / "grid_buffers.hip",
+           CSRC / "grid_rs_pw.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-munsafe-fp-atomics",
@@ -49,8 +50,10 @@ def build(verbose=False, extra=()):
     with ThreadPoolExecutor(workers) as ex:
         warn = [w for ws in ex.map(lambda c: _run(c, verbose), cmds) for w in ws]
     tmp = LIB.with_suffix(".so.tmp")
+    rocm_lib = str(pathlib.Path(HIPCC).resolve().parent.parent / "lib")
     warn += _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
-                  *map(str, objs)], verbose)
+                  *map(str, objs), "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lhipfft"],
+                 verbose)
     if verbose and warn:
         print("\n".join(warn))
     os.replace(tmp, LIB)

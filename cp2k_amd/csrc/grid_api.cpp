@@ -144,6 +144,14 @@ hipStream_t stream() {
   return S.own_stream;
 }
 
+}  // namespace
+
+namespace gm {
+// The stream all library work is launched on (the other sources launch there).
+hipStream_t library_stream() { return stream(); }
+}  // namespace gm
+
+namespace {
 int precision() {
   if (S.precision < 0) {
     const char *e = getenv("GRID_MI355X_PRECISION");
@@ -2044,9 +2052,10 @@ void count_stats(const TaskListImpl *tl, bool collocate, int lpshift) {
 }
 
 int decontract_chunk(const TaskListImpl *tl) {
-  // columns of the Cartesian block per LDS pass: nsb*cach doubles <= 32 KiB
+  // columns of the Cartesian block per LDS pass: nsb*cach doubles <= 8 KiB
+  // per wave (one wave per group, four per workgroup)
   const int nsb = std::max(tl->max_nsb, 1);
-  return std::max(1, std::min(std::max(tl->max_ncoa, 1), 4096 / nsb));
+  return std::max(1, std::min(std::max(tl->max_ncoa, 1), 1024 / nsb));
 }
 
 void run_decontract(TaskListImpl *tl, const double *pab, hipStream_t s,

@@ -747,78 +747,87 @@ __global__ __launch_bounds__(64 * NW) void integrate_kernel(PointParams P) {
 
 // ---------------------------------------------------------------------------
 // Decontraction / contraction of matrix sub-blocks (grid_ref_task_list.c
-// :237-271 and :462-499), one workgroup per (block, iset, jset) group.
-// Columns of the Cartesian block are processed in chunks of `cach`.
+// :237-271 and :462-499), one wave per (block, iset, jset) group (the
+// groups are small: tens of Cartesian functions per side), four groups per
+// workgroup.  Columns of the Cartesian block are processed in chunks of
+// `cach`; each wave owns `wave_doubles` of LDS.
 // ---------------------------------------------------------------------------
+constexpr int kGroupWaves = 4;
+
 __device__ __forceinline__ double block_elem(const Group &g, const double *blk, int sa,
                                              int sb) {
   return g.transpose ? blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa]
                      : blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
 }
 
-__global__ __launch_bounds__(256) void decontract_kernel(const Group *groups,
-                                                         const double *pab, long long pab_len,
-                                                         const double *sphi, double *cart,
-                                                         long long cart_len, int cach) {
-  extern __shared__ double work[];
-  const Group g = groups[blockIdx.x];
-  const int tid = threadIdx.x, nthr = blockDim.x;
+__global__ __launch_bounds__(64 * kGroupWaves) void decontract_kernel(
+    const Group *__restrict__ groups, int ngroups, const double *__restrict__ pab,
+    const double *__restrict__ sphi, double *__restrict__ cart, long long cart_len, int cach,
+    int wave_doubles) {
+  extern __shared__ double lds_dc[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gi = blockIdx.x * kGroupWaves + wave;
+  if (gi >= ngroups) return;
+  const Group &g = groups[gi];
+  double *work = lds_dc + (size_t)wave * wave_doubles;  // [nsb][nc]
   const double *blk = pab + g.block_off;
   for (int c0 = 0; c0 < g.ncoa; c0 += cach) {
     const int nc = min(cach, g.ncoa - c0);
-    for (int e = tid; e < g.nsb * nc; e += nthr) {
+    for (int e = lane; e < g.nsb * nc; e += 64) {
       const int sb = e / nc, ca = c0 + e % nc;
       double s = 0.0;
       for (int sa = 0; sa < g.nsa; sa++)
-        s += block_elem(g, blk, sa, sb) * sphi[g.sphi_a + (long long)sa * g.maxcoa + ca];
+        s = fma(block_elem(g, blk, sa, sb), sphi[g.sphi_a + (long long)sa * g.maxcoa + ca], s);
       work[e] = s;
     }
-    __syncthreads();
-    for (int e = tid; e < g.ncob * nc; e += nthr) {
+    wave_sync();
+    for (int e = lane; e < g.ncob * nc; e += 64) {
       const int cb = e / nc, cc = e % nc;
       double s = 0.0;
       for (int sb = 0; sb < g.nsb; sb++)
-        s += sphi[g.sphi_b + (long long)sb * g.maxcob + cb] * work[sb * nc + cc];
+        s = fma(sphi[g.sphi_b + (long long)sb * g.maxcob + cb], work[sb * nc + cc], s);
       const long long q = g.cart_off + (long long)cb * g.ncoa + c0 + cc;
       if (q < cart_len) cart[q] = s;
     }
-    __syncthreads();
+    wave_sync();
   }
-  (void)pab_len;
 }
 
-__global__ __launch_bounds__(256) void contract_kernel(const Group *groups,
-                                                       const double *hab_cart,
-                                                       const double *sphi, double *hab,
-                                                       long long hab_len, int cach) {
-  extern __shared__ double lds2[];
-  const Group g = groups[blockIdx.x];
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  double *acc = lds2;                    // [nsb][nsa]
-  double *work = lds2 + g.nsa * g.nsb;   // [nsb][cach]
-  for (int e = tid; e < g.nsa * g.nsb; e += nthr) acc[e] = 0.0;
-  __syncthreads();
+__global__ __launch_bounds__(64 * kGroupWaves) void contract_kernel(
+    const Group *__restrict__ groups, int ngroups, const double *__restrict__ hab_cart,
+    const double *__restrict__ sphi, double *__restrict__ hab, long long hab_len, int cach,
+    int wave_doubles, int acc_doubles) {
+  extern __shared__ double lds_ct[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gi = blockIdx.x * kGroupWaves + wave;
+  if (gi >= ngroups) return;
+  const Group &g = groups[gi];
+  double *acc = lds_ct + (size_t)wave * wave_doubles;  // [nsb][nsa]
+  double *work = acc + acc_doubles;                    // [nsb][nc]
+  for (int e = lane; e < g.nsa * g.nsb; e += 64) acc[e] = 0.0;
   for (int c0 = 0; c0 < g.ncoa; c0 += cach) {
     const int nc = min(cach, g.ncoa - c0);
-    for (int e = tid; e < g.nsb * nc; e += nthr) {
+    for (int e = lane; e < g.nsb * nc; e += 64) {
       const int sb = e / nc, cc = e % nc;
       double s = 0.0;
       for (int cb = 0; cb < g.ncob; cb++)
-        s += sphi[g.sphi_b + (long long)sb * g.maxcob + cb] *
-             hab_cart[g.cart_off + (long long)cb * g.ncoa + c0 + cc];
+        s = fma(sphi[g.sphi_b + (long long)sb * g.maxcob + cb],
+                hab_cart[g.cart_off + (long long)cb * g.ncoa + c0 + cc], s);
       work[e] = s;
     }
-    __syncthreads();
-    for (int e = tid; e < g.nsa * g.nsb; e += nthr) {
+    wave_sync();
+    for (int e = lane; e < g.nsa * g.nsb; e += 64) {
       const int sb = e / g.nsa, sa = e % g.nsa;
       double s = 0.0;
       for (int cc = 0; cc < nc; cc++)
-        s += work[sb * nc + cc] * sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + cc];
+        s = fma(work[sb * nc + cc], sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + cc], s);
       acc[e] += s;
     }
-    __syncthreads();
+    wave_sync();
   }
-  for (int e = tid; e < g.nsa * g.nsb; e += nthr) {
+  for (int e = lane; e < g.nsa * g.nsb; e += 64) {
     const int sb = e / g.nsa, sa = e % g.nsa;
     const long long q = g.block_off + (g.transpose ? (long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa
                                                    : (long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb);
@@ -954,9 +963,14 @@ hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab
                              long long pab_len, const double *sphi, double *cart,
                              long long cart_len, int cach, int max_nsb, hipStream_t s) {
   if (ngroups == 0) return hipSuccess;
-  const size_t lds = sizeof(double) * (size_t)cach * (size_t)max_nsb;
-  hipLaunchKernelGGL(decontract_kernel, dim3(ngroups), dim3(256), lds, s, groups, pab,
-                     pab_len, sphi, cart, cart_len, cach);
+  (void)pab_len;
+  const int wave_doubles = (cach * max_nsb + 1) & ~1;  // 16-byte aligned wave regions
+  const size_t lds = sizeof(double) * (size_t)wave_doubles * kGroupWaves;
+  const int blocks = (ngroups + kGroupWaves - 1) / kGroupWaves;
+  const hipError_t e = allow_lds(decontract_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(decontract_kernel, dim3(blocks), dim3(64 * kGroupWaves), lds, s, groups,
+                     ngroups, pab, sphi, cart, cart_len, cach, wave_doubles);
   return hipGetLastError();
 }
 
@@ -964,9 +978,14 @@ hipError_t launch_contract(const Group *groups, int ngroups, const double *hab_c
                            const double *sphi, double *hab, long long hab_len, int cach,
                            int max_nsb, int max_nsa_nsb, hipStream_t s) {
   if (ngroups == 0) return hipSuccess;
-  const size_t lds = sizeof(double) * ((size_t)max_nsa_nsb + (size_t)cach * (size_t)max_nsb);
-  hipLaunchKernelGGL(contract_kernel, dim3(ngroups), dim3(256), lds, s, groups, hab_cart,
-                     sphi, hab, hab_len, cach);
+  const int acc_doubles = (max_nsa_nsb + 1) & ~1;
+  const int wave_doubles = acc_doubles + ((cach * max_nsb + 1) & ~1);
+  const size_t lds = sizeof(double) * (size_t)wave_doubles * kGroupWaves;
+  const int blocks = (ngroups + kGroupWaves - 1) / kGroupWaves;
+  const hipError_t e = allow_lds(contract_kernel, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(contract_kernel, dim3(blocks), dim3(64 * kGroupWaves), lds, s, groups,
+                     ngroups, hab_cart, sphi, hab, hab_len, cach, wave_doubles, acc_doubles);
   return hipGetLastError();
 }
 

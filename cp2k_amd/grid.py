@@ -42,6 +42,8 @@ EXPORTS = [
     "grid_mi355x_pack_buffer", "grid_mi355x_unpack_buffer", "grid_mi355x_buffer_set_resident",
     "grid_mi355x_buffer_sync_host", "grid_mi355x_buffer_host_written",
     "grid_mi355x_buffer_device_current",
+    "grid_mi355x_density_rs2pw", "grid_mi355x_potential_pw2rs", "grid_mi355x_transfer_rs2pw",
+    "grid_mi355x_transfer_pw2rs",
 ]
 
 GRID_FUNC_AB = 100
@@ -122,6 +124,11 @@ def lib():
         L.grid_mi355x_device_free.argtypes = [C.c_void_p]
         L.grid_mi355x_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.grid_mi355x_collocate_pgf_product.restype = None
+        L.grid_mi355x_density_rs2pw.argtypes = [C.c_int, _ip, C.POINTER(C.c_void_p), C.c_void_p,
+                                                C.c_void_p]
+        L.grid_mi355x_potential_pw2rs.argtypes = [C.c_int, _ip, C.c_void_p, C.POINTER(C.c_void_p)]
+        L.grid_mi355x_transfer_rs2pw.argtypes = [_ip, _ip, C.c_void_p, C.c_void_p]
+        L.grid_mi355x_transfer_pw2rs.argtypes = [_ip, _ip, C.c_void_p, C.c_void_p]
         L.grid_mi355x_pack_buffer.argtypes = [C.c_int, C.POINTER(C.c_void_p), _ip, _ip,
                                               C.POINTER(OffloadBuffer)]
         L.grid_mi355x_unpack_buffer.argtypes = [C.c_int, C.POINTER(C.c_void_p), _ip, _ip,
@@ -435,6 +442,58 @@ def unpack_buffer(blocks, pair_offsets, src):
     ptrs, sizes, offs = _block_args(blocks, pair_offsets)
     lib().grid_mi355x_unpack_buffer(C.c_int(len(blocks)), ptrs, _ptr(sizes, C.c_int),
                                     _ptr(offs, C.c_int), src.h)
+
+
+# ---- real-space multigrids <-> plane-wave grids (include/grid_rs_pw.h) ----
+def _npts(shapes):
+    """{nx, ny, nz} per level from [nz, ny, nx] shapes."""
+    return np.ascontiguousarray([list(s)[::-1] for s in shapes], dtype=np.int32)
+
+
+def density_rs2pw(rs_grids, want_g=True):
+    """grid_mi355x_density_rs2pw on DeviceArray level grids ([nz, ny, nx],
+    level 0 the finest): returns (rho_r, rho_g) as DeviceArrays, rho_g of
+    shape [nz, ny, nx, 2] (real, imaginary) or None."""
+    shapes = [g.shape for g in rs_grids]
+    npts = _npts(shapes)
+    rho_r = DeviceArray(shapes[0])
+    rho_g = DeviceArray(tuple(shapes[0]) + (2,)) if want_g else None
+    ptrs = (C.c_void_p * len(rs_grids))(*[g.ptr for g in rs_grids])
+    lib().grid_mi355x_density_rs2pw(C.c_int(len(rs_grids)), _ptr(npts, C.c_int), ptrs,
+                                    C.c_void_p(rho_r.ptr),
+                                    C.c_void_p(rho_g.ptr if rho_g is not None else 0))
+    return rho_r, rho_g
+
+
+def potential_pw2rs(v_r, shapes):
+    """grid_mi355x_potential_pw2rs: DeviceArray level grids of the given
+    [nz, ny, nx] shapes (shapes[0] is v_r's) from the DeviceArray v_r."""
+    out = [DeviceArray(s) for s in shapes]
+    npts = _npts(shapes)
+    ptrs = (C.c_void_p * len(out))(*[g.ptr for g in out])
+    lib().grid_mi355x_potential_pw2rs(C.c_int(len(out)), _ptr(npts, C.c_int),
+                                      C.c_void_p(v_r.ptr), ptrs)
+    return out
+
+
+def transfer_rs2pw(rs, border):
+    """Real region of a halo grid (DeviceArray [nz + 2bz, ny + 2by, nx + 2bx]);
+    border = (bx, by, bz)."""
+    b = np.ascontiguousarray(border, dtype=np.int32)
+    shape = tuple(n - 2 * w for n, w in zip(rs.shape, b[::-1]))
+    pw = DeviceArray(shape)
+    lib().grid_mi355x_transfer_rs2pw(_ptr(_npts([shape])[0], C.c_int), _ptr(b, C.c_int),
+                                     C.c_void_p(rs.ptr), C.c_void_p(pw.ptr))
+    return pw
+
+
+def transfer_pw2rs(pw, border):
+    b = np.ascontiguousarray(border, dtype=np.int32)
+    shape = tuple(n + 2 * w for n, w in zip(pw.shape, b[::-1]))
+    rs = DeviceArray(shape)
+    lib().grid_mi355x_transfer_pw2rs(_ptr(_npts([pw.shape])[0], C.c_int), _ptr(b, C.c_int),
+                                     C.c_void_p(pw.ptr), C.c_void_p(rs.ptr))
+    return rs
 
 
 # ---- single task (include/grid_pgf_product.h), shaped like the Fortran

@@ -214,6 +214,70 @@ def test_buffer_pack_unpack():
     assert all(np.array_equal(a, 2 * b) for a, b in zip(back, blocks))
 
 
+def _band_limited(rng, shape_box, nterms=6):
+    """A real trigonometric sum with Miller vectors strictly inside the box
+    of shape_box (no Nyquist plane): [(h, a, b)]."""
+    lim = [(n - 1) // 2 for n in shape_box[::-1]]  # (x, y, z) of an [nz, ny, nx] shape
+    return [(np.array([rng.integers(-m, m + 1) for m in lim]), rng.normal(), rng.normal())
+            for _ in range(nterms)]
+
+
+def _sample(terms, shape):
+    """The sum at the points of a grid of shape [nz, ny, nx] of one cell:
+    sum a cos(2 pi h.x) + b sin(2 pi h.x), x_d = j_d / n_d (h = (hx, hy, hz))."""
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz) / nz, np.arange(ny) / ny, np.arange(nx) / nx,
+                          indexing="ij")
+    out = np.zeros(shape)
+    for h, a, b in terms:
+        ph = 2 * np.pi * (h[0] * x + h[1] * y + h[2] * z)
+        out += a * np.cos(ph) + b * np.sin(ph)
+    return out
+
+
+@pytest.mark.parametrize("shapes", [[(27, 25, 21), (15, 15, 9), (9, 7, 5)],
+                                    [(24, 20, 18), (12, 10, 8)]])
+def test_rs_pw_closed_forms(shapes):
+    """density_rs2pw of per-level band-limited functions is their sum on the
+    finest grid; potential_pw2rs of a band-limited potential samples it on
+    every level times N_0 / N_l; charge is conserved; and the two are
+    adjoint: <rho(rs), v> = sum_l <rs_l, v_l>."""
+    from oracle import rs_pw
+    rng = np.random.default_rng(len(shapes))
+    coarsest = shapes[-1]
+    per_level = [_band_limited(rng, coarsest) for _ in shapes]
+    rs = [_sample(t, s) for t, s in zip(per_level, shapes)]
+    rho_r, rho_g = rs_pw.density_rs2pw(rs)
+    want = sum(_sample(t, shapes[0]) for t in per_level)
+    assert np.max(np.abs(rho_r - want)) < 1e-12 * np.max(np.abs(want))
+    vt = _band_limited(rng, coarsest)
+    out = rs_pw.potential_pw2rs(_sample(vt, shapes[0]), shapes)
+    for s, o in zip(shapes, out):
+        w = _sample(vt, s) * (np.prod(shapes[0]) / np.prod(s))
+        assert np.max(np.abs(o - w)) < 1e-12 * np.max(np.abs(w))
+    # arbitrary (not band-limited) data
+    rs = [rng.normal(size=s) for s in shapes]
+    rho_r, rho_g = rs_pw.density_rs2pw(rs)
+    N0 = np.prod(shapes[0])
+    assert abs(rho_r.sum() / N0 - sum(g.sum() / g.size for g in rs)) < 1e-12 * N0
+    assert abs(rho_g[0, 0, 0].real - sum(g.mean() for g in rs)) < 1e-13
+    v = rng.normal(size=shapes[0])
+    vl = rs_pw.potential_pw2rs(v, shapes)
+    lhs = float(np.sum(rho_r * v))
+    rhs = sum(float(np.sum(g * w)) for g, w in zip(rs, vl))
+    assert abs(lhs - rhs) < 1e-10 * (abs(lhs) + 1.0)
+
+
+def test_rs_pw_transfer_halo():
+    from oracle import rs_pw
+    rng = np.random.default_rng(3)
+    pw = rng.normal(size=(6, 5, 7))
+    rs = rs_pw.transfer_pw2rs(pw, (2, 1, 3))
+    assert rs.shape == (12, 7, 11)
+    assert np.array_equal(rs_pw.transfer_rs2pw(rs, (2, 1, 3)), pw)
+    assert rs[0, 0, 0] == pw[-3, -1, -2] and rs[-1, -1, -1] == pw[2, 0, 1]
+
+
 # ---- task-list layer ----
 def _tasks_by_hand(args, pab_blocks, func):
     """Collocation of a task list via numpy decontraction + single-task oracle."""
