@@ -186,6 +186,29 @@ def rocprof_summary(d, calls, copy_mib):
             out.append(f"{r['Name'][:70]:70s} {int(r['Calls']):6d} "
                        f"{float(r['TotalDurationNs']) / 1e6:10.3f} "
                        f"{float(r['AverageNs']) / 1e6:9.3f} {float(r['Percentage']):6.2f}")
+    for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)):
+        # dispatches after the calibration copies (issued after autotuning,
+        # right before the timed steps): the steps' kernels only
+        rows = list(csv.DictReader(open(f)))
+        dur = lambda r: int(r["End_Timestamp"]) - int(r["Start_Timestamp"])  # noqa: E731
+        copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        big = max((dur(r) for r in copies), default=0)
+        start = max((int(r["End_Timestamp"]) for r in copies if dur(r) > big // 2), default=0)
+        agg = {}
+        for r in rows:
+            if int(r["Start_Timestamp"]) < start or "copyBuffer" in r["Kernel_Name"]:
+                continue
+            a = agg.setdefault(r["Kernel_Name"], [0, 0.0])
+            a[0] += 1
+            a[1] += dur(r) / 1e6
+        out.append(f"== kernel trace after the calibration copies ({os.path.relpath(f, d)}), "
+                   f"{calls} call(s) of each")
+        out.append(f"{'kernel':70s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s}")
+        for name, (n, ms) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+            out.append(f"{name[:70]:70s} {n:6d} {ms:10.3f} {ms / n:9.4f}")
+        for call in ("collocate", "integrate"):
+            ms = sum(v[1] for k, v in agg.items() if call_of(k) == call)
+            out.append(f"call {call}: kernels {ms / calls:.3f} ms per call")
     per = {}  # counter -> {"copy": [...], call: total KiB}
     for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
         for r in csv.DictReader(open(f)):
