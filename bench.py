@@ -616,6 +616,16 @@ def measure_batch(opt, name):
     def step():
         batch.run(3, forces=forces)
 
+    tuned = None
+    if not opt.no_autotune:
+        # launch shape of the batch's point kernels: one workgroup per unit
+        # or persistent launches over device work queues, the faster kept
+        t0 = time.time()
+        grid.set_precision(grid.FP64)
+        queue = batch.autotune(3, forces=forces)
+        tuned = {"seconds": round(time.time() - t0, 2), "work_queue": queue}
+        log(f"{name}: autotuned in {tuned['seconds']} s (work queue {queue})")
+
     def timed(mode):
         grid.set_precision(mode)
         for _ in range(max(opt.warmup, 1)):
@@ -647,7 +657,7 @@ def measure_batch(opt, name):
         "config": {"workload": WORKLOADS.get(name, name), "config_name": name, "lists": nlists,
                    "ntasks": ntasks, "task_points": points, "task_points_by_lp": by_lp,
                    "tasks_per_s": round(2 * ntasks / (ms_step * 1e-3), 1),
-                   "forces": forces, "generate_s": round(gen_s, 2)},
+                   "forces": forces, "generate_s": round(gen_s, 2), "autotune": tuned},
         "kernel_ms_per_step": {k: round(v[0] / max(opt.steps, 1), 4) for k, v in kt.items()},
         "cpu_baseline": None,
     }

@@ -431,6 +431,11 @@ struct TaskListImpl {
   std::vector<TileInfo> gtiles;                    // tiles of the general tasks
   std::map<int, TilePlan> gtile_plans;             // by lp shift
   DevBuf d_gitems;                                 // their items
+  DevBuf d_queue;                                  // work-queue cursors (merged launches)
+  // merged launches: the tile collocation and the point integration as
+  // persistent launches over device work queues (set per batch, by the
+  // batch autotuner or grid_mi355x_batch_set_work_queue)
+  bool work_queue = false;
   // Per-call level tables.  A call returns once its work is enqueued, so the
   // table of a call still in flight must not be rewritten: each call takes
   // the next slot of a ring (pinned staging + device copy), and waits for
@@ -482,6 +487,7 @@ struct TaskListImpl {
     for (auto &kv : merged_plans) kv.second.d.release();
     for (auto &kv : gtile_plans) kv.second.d.release();
     d_gitems.release();
+    d_queue.release();
     for (DevBuf *b : bufs) b->release();
     for (auto &b : grid_dev) b.release();
     for (LevelSlot &ls : level_slots) {
@@ -1254,7 +1260,9 @@ int lds_limit() {
     int dev = 0, v = 0;
     HIP_CHECK(hipGetDevice(&dev));
     HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    limit = v > 0 ? v : 65536;
+    // minus room for the kernels' static LDS variables (a work-queue slot,
+    // the per-task kernels' task record)
+    limit = (v > 0 ? v : 65536) - 512;
   }
   return limit;
 }
@@ -1446,12 +1454,18 @@ TilePlan &tile_plan(TaskListImpl *tl, int sh, bool general) {
       plan.range[l][r] = std::make_pair(first, (int)units.size() - first);
     }
   }
-  // merged launches: the units of every level, bucket by bucket
+  // merged launches: the units of every level, bucket by bucket, the units
+  // with the most items first (the work queue deals them in this order)
   for (int r = 0; r < 3; r++) {
     const int first = (int)units.size();
     for (int l = 0; l < tl->nlevels; l++)
       for (int u = 0; u < plan.range[l][r].second; u++)
         units.push_back(units[plan.range[l][r].first + u]);
+    auto items_of = [](const TileUnit &u) {
+      return (u.iend - u.ibeg - u.first + u.stride - 1) / u.stride;
+    };
+    std::stable_sort(units.begin() + first, units.end(),
+                     [&](const TileUnit &a, const TileUnit &b) { return items_of(a) > items_of(b); });
     plan.mrange[r] = std::make_pair(first, (int)units.size() - first);
   }
   upload(plan.d, units);
@@ -1796,6 +1810,11 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
       lists[kMOrthoRestC].push_back(t);
     }
   }
+  // the persistent integrate launches take tasks from a work queue in list
+  // order: the largest first, so the smallest fill the tail
+  for (int m : {kMOrthoA, kMOrthoB})
+    std::stable_sort(lists[m].begin(), lists[m].end(),
+                     [&](int a, int b) { return tl->class_npts[a] > tl->class_npts[b]; });
   std::vector<int> all;
   for (int m = 0; m < kMNum; m++) {
     plan.off[m] = (int)all.size();
@@ -1814,6 +1833,21 @@ MergedPlan &merged_plan(TaskListImpl *tl, int sh) {
 void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
   const int sh = P.ld[0] + P.ld[2];
   MergedPlan &mp = merged_plan(tl, sh);
+  // work-queue cursors of the persistent launches (at most one per tile
+  // bucket and level, plus the integrate launches), zeroed by a kernel (no
+  // memset node in a captured graph); the buffer is sized once per list
+  const int slots = 2 * (3 * tl->nlevels + 8);
+  int *cursors = nullptr;
+  int next_slot = 0;
+  if (tl->work_queue) {
+    cursors = (int *)tl->d_queue.get(sizeof(int) * slots);
+    HIP_CHECK(launch_fill((double *)cursors, slots / 2, 0.0, s));
+  }
+  auto queue_of = [&]() -> int * {
+    if (!cursors) return nullptr;
+    if (next_slot >= slots) fatal("merged call: more than %d persistent launches", slots);
+    return cursors + next_slot++;
+  };
   const int *idx = (const int *)mp.d.p;
   auto sub = [&](int m) {
     PointParams Q = P;
@@ -1848,6 +1882,8 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         if (R.ntasks <= 0) continue;
         const int lpb = m == kMOrthoA ? 4 : 6;
         R.wp = make_wave_plan(tl, -1, std::min(lpb, mp.lp[m]), P.ld, kPlanPoints);
+        R.queue = queue_of();
+        R.qchunk = 4;
         TimedLaunch tm(9, s);
         HIP_CHECK(launch_integrate_wave(R, lpb, s));
       }
@@ -1871,6 +1907,7 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         if (8 * max_window + 4 * wave <= lds_limit()) {
           R.unit_base = tp.mrange[r].first;
           R.wp = make_wave_plan(tl, -1, lp, P.ld, kPlanWindow);
+          R.queue = queue_of();
           TimedLaunch tm(7, s);
           HIP_CHECK(launch_collocate_tiles(R, lpb, tp.mrange[r].second, s));
         } else {
@@ -1880,6 +1917,7 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
             if (!tl->olev[l].tiled || tp.range[l][r].second <= 0) continue;
             R.unit_base = tp.range[l][r].first;
             R.wp = make_wave_plan(tl, l, lp, P.ld, kPlanWindow);
+            R.queue = queue_of();
             TimedLaunch tm(7, s);
             HIP_CHECK(launch_collocate_tiles(R, lpb, tp.range[l][r].second, s));
           }
@@ -2313,9 +2351,9 @@ struct BatchImpl {
 
 // Device buffers of the merged list that a captured graph addresses and that
 // a call may (re)allocate on demand; a graph is valid while none has moved.
-std::array<const void *, 6> captured_buffers(const TaskListImpl *tl) {
+std::array<const void *, 7> captured_buffers(const TaskListImpl *tl) {
   return {tl->d_cxyz.p, tl->d_gridf.p, tl->d_cart.p, tl->d_hab_cart.p, tl->d_task_out.p,
-          tl->d_gitems.p};
+          tl->d_gitems.p, tl->d_queue.p};
 }
 
 BatchImpl *checked_batch(const void *p) {
@@ -3449,6 +3487,52 @@ void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what, const enum 
 void grid_mi355x_batch_task_points(const grid_mi355x_batch *batch, long long *points) {
   const BatchImpl *b = checked_batch(batch);
   *points = grid_mi355x_task_points((const grid_task_list *)b->merged);
+}
+
+void grid_mi355x_batch_set_work_queue(grid_mi355x_batch *batch, const bool enable) {
+  BatchImpl *b = checked_batch(batch);
+  if (b->merged->work_queue == enable) return;
+  sync_stream(stream());
+  b->drop_graphs();  // the captured launch shapes change
+  b->merged->work_queue = enable;
+}
+
+int grid_mi355x_batch_work_queue(const grid_mi355x_batch *batch) {
+  return checked_batch(batch)->merged->work_queue ? 1 : 0;
+}
+
+// Batch autotuner: times the run (what, func, compute_tau, forces) as
+// graph replays with the point kernels launched one workgroup per unit and
+// as persistent launches over device work queues, and keeps the faster.
+int grid_mi355x_autotune_batch(grid_mi355x_batch *batch, const int what, const enum grid_func func,
+                               const bool compute_tau, const bool forces) {
+  BatchImpl *b = checked_batch(batch);
+  if (b->merged->empty) return 0;
+  const hipStream_t s = stream();
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  float best = -1.0f;
+  int choice = 0;
+  for (int q = 0; q < 2; q++) {
+    grid_mi355x_batch_set_work_queue(batch, q == 1);
+    grid_mi355x_batch_run(batch, what, func, compute_tau, forces);  // capture
+    HIP_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; r++) grid_mi355x_batch_run(batch, what, func, compute_tau, forces);
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (best < 0.0f || ms < best) {
+      best = ms;
+      choice = q;
+    }
+  }
+  grid_mi355x_batch_set_work_queue(batch, choice == 1);
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  sync_stream(s);
+  return choice;
 }
 
 

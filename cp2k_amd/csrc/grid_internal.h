@@ -181,6 +181,13 @@ struct PointParams {
   // where set
   double *pgf_out;
   double *hdab, *hadb, *a_hdab;
+  // device work queue (persistent launches of batched lists): if set, the
+  // launch's workgroups (waves) take units (groups of qchunk tasks) from the
+  // atomic cursor *queue, zeroed before the launch, until nunits (ntasks)
+  // are taken
+  int *queue;
+  int qchunk;
+  int nunits;
 };
 
 // Task of position t of a launch.

@@ -922,7 +922,18 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
-  const TileUnit U = units[P.unit_base + blockIdx.x];
+  // one unit per workgroup, or (work queue) units from the cursor until
+  // all are taken
+  __shared__ int next_unit;
+  int u = blockIdx.x;
+  if (P.queue) {
+    if (threadIdx.x == 0) next_unit = atomicAdd(P.queue, 1);
+    __syncthreads();
+    u = next_unit;
+    __syncthreads();
+  }
+  while (u < P.nunits) {
+  const TileUnit U = units[P.unit_base + u];
   const Level &L = levels[U.level];
   const Layout &lay = L.lay;
   const int window = lay.npts_local[0] * U.by * U.bz;  // <= wp.window
@@ -966,6 +977,13 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
       if (v != 0.0) atomicAdd(&dst[x], v);
     }
   }
+  if (!P.queue) break;
+  __syncthreads();  // the window is cleared for the next unit
+  if (threadIdx.x == 0) next_unit = atomicAdd(P.queue, 1);
+  __syncthreads();
+  u = next_unit;
+  __syncthreads();
+  }
 }
 
 // R = float reads the FP32 copy of the potential (Level::gridf).
@@ -976,32 +994,49 @@ __global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel
   const WavePlan &wp = P.wp;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
-  if (t >= P.ntasks) return;
-  const long long task = launch_task(P, t);
-  const Task &T = tasks[task];
-  const GeoClass &G = P.geo[T.geo];
-  const Level &L = levels[T.level];
-  const Layout &lay = L.lay;
+  // one task per wave, or (work queue) groups of qchunk tasks from the
+  // cursor until all are taken
+  long long t0 = (long long)blockIdx.x * (blockDim.x >> 6) + wave, t1 = t0 + 1;
+  if (P.queue) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(P.queue, P.qchunk);
+    t0 = __builtin_amdgcn_readfirstlane(base);
+    t1 = t0 + P.qchunk;
+  }
   const WaveLds W = wave_lds(lds, wp, wave);
-  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
   R *tab = reinterpret_cast<R *>(W.u);
-  wave_tables(T, lay, lp, wp, tab, lane);
-  double *out = P.cxyz_all + task * P.cxyz_stride;
-  const R *grid;
-  if constexpr (sizeof(R) == 4)
-    grid = L.gridf;
-  else
-    grid = L.grid;
-  switch (lp) {
-    case 0: integrate_task<0>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    case 1: integrate_task<1>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    case 2: integrate_task<2>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    case 3: integrate_task<3>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    case 4: integrate_task<4>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
-    default: break;  // the host routes larger lp to the per-task kernels
+  while (t0 < P.ntasks) {
+    for (long long t = t0; t < t1 && t < P.ntasks; t++) {
+      const long long task = launch_task(P, t);
+      const Task &T = tasks[task];
+      const GeoClass &G = P.geo[T.geo];
+      const Level &L = levels[T.level];
+      const Layout &lay = L.lay;
+      const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+      wave_tables(T, lay, lp, wp, tab, lane);
+      double *out = P.cxyz_all + task * P.cxyz_stride;
+      const R *grid;
+      if constexpr (sizeof(R) == 4)
+        grid = L.gridf;
+      else
+        grid = L.grid;
+      switch (lp) {
+        case 0: integrate_task<0>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        case 1: integrate_task<1>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        case 2: integrate_task<2>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        case 3: integrate_task<3>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        case 4: integrate_task<4>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
+        default: break;  // the host routes larger lp to the per-task kernels
+      }
+      wave_sync();  // the tables are rewritten for the next task
+    }
+    if (!P.queue) break;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(P.queue, P.qchunk);
+    t0 = __builtin_amdgcn_readfirstlane(base);
+    t1 = t0 + P.qchunk;
   }
 }
 
@@ -1095,6 +1130,26 @@ static hipError_t launch_wave(K kernel, const PointParams &P, int blocks, hipStr
 
 static int wave_blocks(const PointParams &P) { return (P.ntasks + P.wp.nwaves - 1) / P.wp.nwaves; }
 
+// Workgroups of a persistent launch (work queue): as many as are resident at
+// once on the device (occupancy of the kernel with its LDS), at most `cap`.
+template <typename K>
+static int persistent_blocks(K kernel, int threads, size_t lds, int cap) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 1;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kernel),
+                                                   threads, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const long long n = (long long)per_cu * cus;
+  return (int)(n < cap ? n : cap);
+}
+
 hipError_t launch_cxyz_lane(const PointParams &P, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   hipLaunchKernelGGL(cxyz_lane_kernel, dim3((P.ntasks + 255) / 256), dim3(256), 0, s, P.tasks,
@@ -1125,20 +1180,24 @@ hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s) {
 
 template <typename R>
 static hipError_t collocate_tiles_r(const PointParams &P, int lpb, int nunits, hipStream_t s) {
-  if (lpb == 4)
-    return launch_wave(collocate_tile_kernel<4, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
-  if (lpb == 6)
-    return launch_wave(collocate_tile_kernel<6, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
-  if (lpb == 8)
-    return launch_wave(collocate_tile_kernel<8, R>, P, nunits, s, P.tasks, P.levels, P.units, P.items);
+  auto go = [&](auto kernel) {
+    const int blocks = P.queue ? persistent_blocks(kernel, 64 * P.wp.nwaves, P.wp.bytes, nunits)
+                               : nunits;
+    return launch_wave(kernel, P, blocks, s, P.tasks, P.levels, P.units, P.items);
+  };
+  if (lpb == 4) return go(collocate_tile_kernel<4, R>);
+  if (lpb == 6) return go(collocate_tile_kernel<6, R>);
+  if (lpb == 8) return go(collocate_tile_kernel<8, R>);
   return hipErrorInvalidValue;
 }
 
 // nunits workgroups from unit P.unit_base.
-hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s) {
+hipError_t launch_collocate_tiles(const PointParams &Pin, int lpb, int nunits, hipStream_t s) {
   if (nunits <= 0) return hipSuccess;
-  if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0)
+  if (Pin.wp.nwaves < 1 || Pin.wp.nwaves > (lpb <= 4 ? 12 : 8) || Pin.wp.window <= 0)
     return hipErrorInvalidValue;
+  PointParams P = Pin;
+  P.nunits = nunits;
   count_launch(7, P.fp32);
   return P.fp32 ? collocate_tiles_r<float>(P, lpb, nunits, s)
                 : collocate_tiles_r<double>(P, lpb, nunits, s);
@@ -1146,8 +1205,14 @@ hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hip
 
 template <typename R>
 static hipError_t integrate_wave_r(const PointParams &P, int lpb, hipStream_t s) {
-  if (lpb == 4) return launch_wave(integrate_wave_kernel<4, R>, P, wave_blocks(P), s, P.tasks, P.levels);
-  if (lpb == 6) return launch_wave(integrate_wave_kernel<6, R>, P, wave_blocks(P), s, P.tasks, P.levels);
+  auto go = [&](auto kernel) {
+    const int blocks = P.queue ? persistent_blocks(kernel, 64 * P.wp.nwaves, P.wp.bytes,
+                                                   wave_blocks(P))
+                               : wave_blocks(P);
+    return launch_wave(kernel, P, blocks, s, P.tasks, P.levels);
+  };
+  if (lpb == 4) return go(integrate_wave_kernel<4, R>);
+  if (lpb == 6) return go(integrate_wave_kernel<6, R>);
   return hipErrorInvalidValue;
 }
 

@@ -43,7 +43,8 @@ EXPORTS = [
     "grid_mi355x_buffer_sync_host", "grid_mi355x_buffer_host_written",
     "grid_mi355x_buffer_device_current",
     "grid_mi355x_density_rs2pw", "grid_mi355x_potential_pw2rs", "grid_mi355x_transfer_rs2pw",
-    "grid_mi355x_transfer_pw2rs",
+    "grid_mi355x_transfer_pw2rs", "grid_mi355x_batch_set_work_queue",
+    "grid_mi355x_batch_work_queue", "grid_mi355x_autotune_batch",
 ]
 
 GRID_FUNC_AB = 100
@@ -119,6 +120,9 @@ def lib():
         L.grid_mi355x_batch_grid.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.grid_mi355x_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_bool, C.c_bool]
         L.grid_mi355x_batch_task_points.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+        L.grid_mi355x_batch_set_work_queue.argtypes = [C.c_void_p, C.c_bool]
+        L.grid_mi355x_batch_work_queue.argtypes = [C.c_void_p]
+        L.grid_mi355x_autotune_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_bool, C.c_bool]
         L.grid_mi355x_device_malloc.restype = C.c_void_p
         L.grid_mi355x_device_malloc.argtypes = [C.c_size_t]
         L.grid_mi355x_device_free.argtypes = [C.c_void_p]
@@ -680,6 +684,20 @@ class GridBatch:
     def run(self, what=3, func=GRID_FUNC_AB, compute_tau=False, forces=False):
         lib().grid_mi355x_batch_run(self.h, C.c_int(what), C.c_int(func), C.c_bool(compute_tau),
                                     C.c_bool(forces))
+
+    def set_work_queue(self, enable):
+        """Persistent launches over device work queues (True) or one
+        workgroup per unit (False) for the batch's point kernels."""
+        lib().grid_mi355x_batch_set_work_queue(self.h, C.c_bool(enable))
+
+    def work_queue(self):
+        return bool(lib().grid_mi355x_batch_work_queue(self.h))
+
+    def autotune(self, what=3, func=GRID_FUNC_AB, compute_tau=False, forces=False):
+        """grid_mi355x_autotune_batch: keeps the faster launch shape; True if
+        the work queue won."""
+        return bool(lib().grid_mi355x_autotune_batch(self.h, C.c_int(what), C.c_int(func),
+                                                     C.c_bool(compute_tau), C.c_bool(forces)))
 
     def outputs(self, i):
         """(rho levels, hab, forces, virial) of list i, on the host."""

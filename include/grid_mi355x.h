@@ -169,6 +169,23 @@ void grid_mi355x_batch_run(grid_mi355x_batch *batch, const int what,
 /* Task points of all lists of the batch. */
 void grid_mi355x_batch_task_points(const grid_mi355x_batch *batch, long long *points);
 
+/* Launch shape of a batch's point kernels: enable = true runs the tile
+ * collocation and the point integration as persistent launches (as many
+ * workgroups as are resident at once) that take their units -- tiles of any
+ * list and level, or groups of tasks, the largest first -- from a device
+ * work queue (an atomic cursor, zeroed by a kernel inside the graph);
+ * false (the default) launches one workgroup per unit.  Changing it drops
+ * the captured graphs. */
+void grid_mi355x_batch_set_work_queue(grid_mi355x_batch *batch, const bool enable);
+int grid_mi355x_batch_work_queue(const grid_mi355x_batch *batch);
+
+/* Batch autotuner: times the run (what, func, compute_tau, forces) both
+ * ways (3 graph replays each) and keeps the faster; returns 1 if the work
+ * queue won.  Synchronous. */
+int grid_mi355x_autotune_batch(grid_mi355x_batch *batch, const int what,
+                               const enum grid_func func, const bool compute_tau,
+                               const bool forces);
+
 /* Launch-shape autotuner: for every level that collocates through LDS tiles,
  * times the collocation (func, pab_blocks and grids as in
  * grid_mi355x_collocate_task_list_dev) for each workgroup count (2, 4 or 8

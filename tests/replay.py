@@ -52,3 +52,16 @@ def rel_err(test, ref):
     if ref.size == 0:
         return 0.0
     return float(np.max(np.abs(test - ref) / np.maximum(1.0, np.abs(ref))))
+
+
+def block_weights(args):
+    """Per element of the pab/hab buffer of a generated list: 1 for a block
+    of one atom with itself, 2 for a block of two atoms (the off-site rscale
+    of collocate, src/grid/ref/grid_ref_task_list.c:369; hab is unscaled)."""
+    import numpy as np
+    w = np.zeros(args.buffer_size)
+    bi, bj = args.meta["block_i"], args.meta["block_j"]
+    ends = list(args.block_offsets[1:]) + [args.buffer_size]
+    for b in range(args.nblocks):
+        w[int(args.block_offsets[b]):int(ends[b])] = 1.0 if bi[b] == bj[b] else 2.0
+    return w

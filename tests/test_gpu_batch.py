@@ -181,3 +181,70 @@ def test_batch_graphs_alternate_precision(lists):
                 assert replay.rel_err(v, w[2]) < FTOL, (step, i)
     finally:
         grid.set_precision(grid.FP64)
+
+
+def test_batch_work_queue(lists):
+    """The same batch with its point kernels as persistent launches over
+    device work queues: every replay equals the lists alone; switching the
+    launch shape back and forth (and the autotuner's choice) keeps results."""
+    want = [eager(a, p, V) for a, p, V in lists]
+    batch = grid.GridBatch([grid.GridTaskList(a) for a, _, _ in lists])
+    for i, (a, p, V) in enumerate(lists):
+        batch.set_inputs(i, pab=p, V=V)
+    for step, queue in enumerate([True, True, False, True]):
+        print("batch work queue", step, queue, flush=True)
+        batch.set_work_queue(queue)
+        assert batch.work_queue() == queue
+        batch.run(3, forces=True)
+        grid.synchronize()
+        for i in range(len(lists)):
+            check(batch.outputs(i), want[i], (step, queue, i), tol_of(i, lists))
+    chosen = batch.autotune(3, forces=True)
+    assert batch.work_queue() == chosen
+    batch.run(3, forces=True)
+    grid.synchronize()
+    for i in range(len(lists)):
+        check(batch.outputs(i), want[i], ("tuned", chosen, i), tol_of(i, lists))
+
+
+def test_batch_32_lists_against_oracle():
+    """32 lists at the batched workload's shapes (8-64 atoms, cubic 6-14 A
+    cells, one level of 32^3-112^3 points, l <= 2) in one batch with the work
+    queue, each list against the oracle."""
+    lists = synth.batch_config(32, seed=11, atoms=(8, 64))
+    batch = grid.GridBatch([grid.GridTaskList(a) for a, _, _ in lists])
+    for i, (a, p, V) in enumerate(lists):
+        batch.set_inputs(i, pab=p, V=V)
+    batch.set_work_queue(True)
+    batch.run(3, forces=True)
+    grid.synchronize()
+    for i, (a, p, V) in enumerate(lists):
+        print("batch32 list", i, a.ntasks, flush=True)
+        rho, hab, f, v = batch.outputs(i)
+        ot = O.TaskList(a)
+        for g, w in zip(rho, ot.collocate(100, p)):
+            assert replay.rel_err(g, w) < TOL, i
+        h2, f2, v2 = ot.integrate(V, p, forces=True, virial=True)
+        assert replay.rel_err(hab, h2) < TOL, i
+        assert replay.rel_err(f, f2) < FTOL, i
+        assert replay.rel_err(v, v2) < FTOL, i
+
+
+def test_batch_256_adjoint_identity():
+    """The full 256-list batched workload: per list, the density collocated
+    from P and the hab integrated from the potential V satisfy
+    sum_g rho[P] V = sum_blocks s_ij <P, H[V]> (s = 2 off-diagonal), the
+    adjoint identity of collocate and integrate."""
+    import numpy as np
+    lists = synth.batch_config(256, seed=0)
+    batch = grid.GridBatch([grid.GridTaskList(a) for a, _, _ in lists])
+    for i, (a, p, V) in enumerate(lists):
+        batch.set_inputs(i, pab=p, V=V)
+    batch.set_work_queue(True)
+    batch.run(3)
+    grid.synchronize()
+    for i, (a, p, V) in enumerate(lists):
+        rho, hab, _, _ = batch.outputs(i)
+        lhs = sum(float(np.sum(r * v)) for r, v in zip(rho, V))
+        rhs = float(np.dot(replay.block_weights(a) * p, hab))
+        assert abs(lhs - rhs) <= 1e-10 * (abs(lhs) + 1.0), (i, lhs, rhs)
