@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = [CSRC / "grid_api.cpp", CSRC / "grid_kernels.hip", CSRC / "grid_wave.hip",
            CSRC / "grid_general.hip", CSRC / "grid_noalloc.hip", CSRC / "grid_buffers.hip",
-           CSRC / "grid_rs_pw.hip"]
+           CSRC / "grid_rs_pw.hip", CSRC / "grid_task_gen.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-munsafe-fp-atomics",

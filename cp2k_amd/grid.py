@@ -45,6 +45,7 @@ EXPORTS = [
     "grid_mi355x_density_rs2pw", "grid_mi355x_potential_pw2rs", "grid_mi355x_transfer_rs2pw",
     "grid_mi355x_transfer_pw2rs", "grid_mi355x_batch_set_work_queue",
     "grid_mi355x_batch_work_queue", "grid_mi355x_autotune_batch",
+    "grid_mi355x_generate_task_list", "grid_mi355x_generate_task_list_ms",
 ]
 
 GRID_FUNC_AB = 100
@@ -142,6 +143,8 @@ def lib():
         L.grid_mi355x_buffer_host_written.argtypes = [C.POINTER(OffloadBuffer)]
         L.grid_mi355x_buffer_device_current.argtypes = [C.POINTER(OffloadBuffer)]
         L.grid_mi355x_integrate_pgf_product.restype = None
+        L.grid_mi355x_generate_task_list.restype = C.c_longlong
+        L.grid_mi355x_generate_task_list_ms.restype = C.c_double
         _lib = L
     return _lib
 
@@ -498,6 +501,28 @@ def transfer_pw2rs(pw, border):
     lib().grid_mi355x_transfer_pw2rs(_ptr(_npts([pw.shape])[0], C.c_int), _ptr(b, C.c_int),
                                      C.c_void_p(pw.ptr), C.c_void_p(rs.ptr))
     return rs
+
+
+# ---- task-list generation (include/grid_task_gen.h) ----
+def generate_task_list(nl):
+    """Task list of neighbour list nl (tasklist.NeighbourList), generated on
+    the device: dict of the grid_create_task_list arrays (reference order),
+    nblocks, block_offsets, buffer_size, ntasks and device_ms (device time of
+    the generation)."""
+    from .tasklist import call_generate
+    bases = [Basis(b) for b in nl.basis_sets]
+    arr = (C.POINTER(GridBasisSet) * len(bases))(*[b.h for b in bases])
+    out = call_generate(lib().grid_mi355x_generate_task_list, nl, arr)
+    out["device_ms"] = float(lib().grid_mi355x_generate_task_list_ms())
+    return out
+
+
+def create_generated_task_list(nl, gen=None):
+    """(GridTaskList, TaskListArgs) of neighbour list nl from the device
+    generator."""
+    gen = generate_task_list(nl) if gen is None else gen
+    args = nl.task_list_args(gen)
+    return GridTaskList(args), args
 
 
 # ---- single task (include/grid_pgf_product.h), shaped like the Fortran

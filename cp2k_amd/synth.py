@@ -24,7 +24,7 @@ import sys
 
 import numpy as np
 
-from .tasklist import BasisSpec, TaskListArgs, ncoset
+from .tasklist import BasisSpec, NeighbourList, TaskListArgs, ncoset
 
 ANGSTROM = 1.0 / 0.52917721067  # bohr per angstrom
 
@@ -353,6 +353,17 @@ def build_task_list(cell, orthorhombic, positions, kinds, bases, npts, dh, dh_in
         border_width=np.zeros((nlevels, 3)) if border_width is None else border_width,
         dh=dh, dh_inv=dh_inv, meta=dict(meta or {}))
     args.meta.update(nblocks=nblocks, block_i=bi + 1, block_j=bj + 1)
+    maxset = max(b.nset for b in bases)
+    maxpgf = max(b.maxpgf for b in bases)
+    pgf_r = np.zeros((len(bases), maxset, maxpgf))
+    for k, r in enumerate(rp_kind):
+        pgf_r[k, :r.shape[0], :r.shape[1]] = r
+    args.meta["nl"] = NeighbourList(
+        orthorhombic=orthorhombic, basis_sets=bases, atom_positions=positions,
+        atom_kinds=kinds, pair_atoms=np.stack([pi + 1, pj + 1], axis=1), pair_rab=prab,
+        pair_image=None, set_radius=pgf_r.max(axis=2), pgf_radius=pgf_r, kind_radius=None,
+        cutoff=np.asarray(cutoff, dtype=np.float64), rel_cutoff=float(rel_cutoff),
+        eps_rho=float(eps_rho), npts_global=npts, dh=dh, dh_inv=dh_inv)
     return args
 
 

@@ -164,6 +164,30 @@ void oracle_integrate_task_list_with(const oracle_task_list *tl,
  * (collocate) -- the task-point count used by the throughput metric. */
 long oracle_task_list_points(const oracle_task_list *tl, int lp_extra);
 
+/* ---- task-list generation (oracle/task_gen.c) ----------------------------
+ * src/task_list_methods.F generate_qs_task_list for replicated grids; the
+ * arguments mean what they mean for grid_mi355x_generate_task_list
+ * (include/grid_task_gen.h).  Pinned by closed forms of exp_radius (l = 0
+ * and the bracket property), the binomial expansion of the polynomial
+ * prefactor, gridlevel cases and a brute-force screening count. */
+double oracle_exp_radius(int l, double alpha, double threshold, double prefactor,
+                         double epsabs, double rlow);
+double oracle_exp_radius_bracket(int l, double alpha, double threshold, double prefactor,
+                                 double epsabs, double rlow, double *lo_out);
+void oracle_product_polycoef(int la_max, int lb_max, double rad_a, double rad_b, double *poly);
+double oracle_product_radius(int la_max, int lb_max, const double ra[3], const double rb[3],
+                             const double rp[3], double zetp, double eps, double prefactor);
+int oracle_gridlevel(int nlevels, const double *cutoff, double rel_cutoff, double exponent);
+long long oracle_generate_task_list(
+    int npairs, const int *pair_atoms, const double *pair_rab, const int *pair_image,
+    int natoms, const int *atom_kinds, const double *atom_positions, int nkinds,
+    const oracle_basis *const *basis_sets, int maxset, int maxpgf, const double *set_radius,
+    const double *pgf_radius, const double *kind_radius, int nlevels, const double *cutoff,
+    double rel_cutoff, double eps_rho, long long capacity, int *level_list, int *iatom_list,
+    int *jatom_list, int *iset_list, int *jset_list, int *ipgf_list, int *jpgf_list,
+    int *border_mask_list, int *block_num_list, double *radius_list, double *rab_list,
+    int *nblocks, int *block_offsets, long long *buffer_size);
+
 #ifdef __cplusplus
 }
 #endif

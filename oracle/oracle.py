@@ -241,3 +241,39 @@ class TaskList:
 
 def threads():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+# ---- task-list generation (oracle/task_gen.c) ----
+def generate_task_list(nl):
+    """Oracle restatement of the device task-list generator for
+    cp2k_amd.tasklist.NeighbourList nl (same output dict)."""
+    from cp2k_amd.tasklist import call_generate
+    structs = [make_basis_struct(b) for b in nl.basis_sets]
+    arr = (C.POINTER(OracleBasis) * len(structs))(*[C.pointer(s) for s in structs])
+    return call_generate(lib().oracle_generate_task_list, nl, arr)
+
+
+def exp_radius(l, alpha, threshold, prefactor, epsabs=1.0e-2, rlow=0.0, bracket=False):
+    """exp_radius (src/aobasis/ao_util.F:95-190); bracket=True returns
+    (lower, upper) ends of the final bracket."""
+    L = lib()
+    L.oracle_exp_radius_bracket.restype = C.c_double
+    lo = C.c_double(0.0)
+    hi = L.oracle_exp_radius_bracket(C.c_int(l), C.c_double(alpha), C.c_double(threshold),
+                                     C.c_double(prefactor), C.c_double(epsabs),
+                                     C.c_double(rlow), C.byref(lo))
+    return (lo.value, hi) if bracket else hi
+
+
+def product_polycoef(la_max, lb_max, rad_a, rad_b):
+    poly = np.zeros(la_max + lb_max + 1)
+    lib().oracle_product_polycoef(C.c_int(la_max), C.c_int(lb_max), C.c_double(rad_a),
+                                  C.c_double(rad_b), _ptr(poly))
+    return poly
+
+
+def gridlevel(cutoff, rel_cutoff, exponent):
+    c = _d(cutoff)
+    lib().oracle_gridlevel.restype = C.c_int
+    return lib().oracle_gridlevel(C.c_int(len(c)), _ptr(c), C.c_double(rel_cutoff),
+                                  C.c_double(exponent))
