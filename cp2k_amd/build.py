@@ -52,7 +52,8 @@ def build(verbose=False, extra=()):
     tmp = LIB.with_suffix(".so.tmp")
     rocm_lib = str(pathlib.Path(HIPCC).resolve().parent.parent / "lib")
     warn += _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
-                  *map(str, objs), "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lhipfft"],
+                  *map(str, objs), "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib, "-lhipfft",
+                  "-Wl,-z,defs"],
                  verbose)
     if verbose and warn:
         print("\n".join(warn))

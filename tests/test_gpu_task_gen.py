@@ -12,6 +12,8 @@ defining property of exp_radius_very_extended on its own (tests/gen_brute.py).
 A list built from the generated arrays collocates and integrates like the
 oracle on the same arguments.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -79,12 +81,27 @@ def test_generate_empty_and_repeat():
         np.testing.assert_array_equal(b[k], c[k], err_msg=k)
 
 
+def order_noise(args, pab):
+    """Per level, how far the oracle's own FP64 density moves when the tasks
+    are summed in reverse order."""
+    rev = copy.copy(args)
+    perm = np.arange(args.ntasks)[::-1].copy()
+    for k in ("level_list", "iatom_list", "jatom_list", "iset_list", "jset_list", "ipgf_list",
+              "jpgf_list", "border_mask_list", "block_num_list", "radius_list", "rab_list"):
+        setattr(rev, k, np.ascontiguousarray(getattr(args, k)[perm]))
+    a = O.TaskList(args).collocate(grid.GRID_FUNC_AB, pab)
+    b = O.TaskList(rev).collocate(grid.GRID_FUNC_AB, pab)
+    return a, [replay.rel_err(x, y) for x, y in zip(a, b)]
+
+
 @pytest.mark.parametrize("name,ortho,tol", [("tri10", False, 1e-11), ("h2o6", True, 1e-12)])
 def test_generated_list_collocates(name, ortho, tol):
     """The generated arrays feed grid_create_task_list: collocation and
-    integration of the list match the oracle's on the same arguments (the
+    integration of the list match the oracle's on the same arguments.  The
     10-atom triclinic cell sums thousands of periodic images per point with
-    cancellation: 1e-11, as in the other parity tests)."""
+    cancellation; on its coarsest level the oracle itself moves by about
+    2e-11 when the tasks are summed in another order, so each level is held
+    to the larger of the nominal bar and 4x that noise."""
     print("generated list", name, flush=True)
     args0, _, _ = synth.config(name, seed=2)
     nl = args0.meta["nl"]
@@ -94,9 +111,9 @@ def test_generated_list_collocates(name, ortho, tol):
     pab = synth.random_density(args, rng)
     ol = O.TaskList(args)
     g = gl.collocate(grid.GRID_FUNC_AB, pab)
-    o = ol.collocate(grid.GRID_FUNC_AB, pab)
-    for a, b in zip(g, o):
-        assert replay.rel_err(a, b) < tol
+    o, noise = order_noise(args, pab)
+    for a, b, nz in zip(g, o, noise):
+        assert replay.rel_err(a, b) < max(tol, 4 * nz)
     V = synth.random_potential(args, rng)
     hab, f, v = gl.integrate(V, pab, forces=True, virial=True)
     ho, fo, vo = ol.integrate(V, pab, forces=True, virial=True)
