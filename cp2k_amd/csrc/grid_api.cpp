@@ -36,6 +36,12 @@ hipError_t launch_integrate(const PointParams &P, bool ortho, hipStream_t s);
 hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,
                              long long pab_len, const double *sphi, double *cart,
                              long long cart_len, int cach, int max_nsb, hipStream_t s);
+hipError_t launch_decontract_staged(const Group *groups, int ngroups, const double *pab,
+                                    const double *sphi, double *cart, long long cart_len, int nc,
+                                    int wave_doubles, int nwaves, int blocks, hipStream_t s);
+hipError_t launch_contract_staged(const Group *groups, int ngroups, const double *hab_cart,
+                                  const double *sphi, double *hab, long long hab_len, int nc,
+                                  int wave_doubles, int nwaves, int blocks, hipStream_t s);
 hipError_t launch_contract(const Group *groups, int ngroups, const double *hab_cart,
                            const double *sphi, double *hab, long long hab_len, int cach,
                            int max_nsb, int max_nsa_nsb, hipStream_t s);
@@ -453,7 +459,7 @@ struct TaskListImpl {
   int max_la = 0, max_lb = 0, max_lp = 0, max_cmax = 0, max_rows_ortho = 0;
   int max_range = 0, max_rows_general = 0, max_planes = 0;
   bool has_ortho = false, has_general = false;
-  int max_ncoa = 0, max_nsb = 0, max_nsa_nsb = 0;
+  int max_ncoa = 0, max_nsb = 0, max_nsa_nsb = 0, max_nsa = 0, max_ncob = 0;
   long long block_end = 0;  // max over groups of block offset + block size
   long n_ortho = 0, n_general = 0, n_skip = 0;
   long hist[2][kCounterLp] = {};  // [ortho?][lp of the basis sets]
@@ -761,7 +767,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     sphi_off[k] = (long long)sphi_all.size();
     sphi_all.insert(sphi_all.end(), b->sphi, b->sphi + (size_t)b->nsgf * b->maxco);
   }
-  tl->max_ncoa = tl->max_nsb = tl->max_nsa_nsb = 0;
+  tl->max_ncoa = tl->max_nsb = tl->max_nsa_nsb = tl->max_nsa = tl->max_ncob = 0;
   tl->block_end = 0;
   for (int t = 0; t < ntasks; t++) {
     const int g = (int)(std::lower_bound(ukeys.begin(), ukeys.end(), keys[t]) - ukeys.begin());
@@ -798,6 +804,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     tl->max_ncoa = std::max(tl->max_ncoa, G.ncoa);
     tl->max_nsb = std::max(tl->max_nsb, G.nsb);
     tl->max_nsa_nsb = std::max(tl->max_nsa_nsb, G.nsa * G.nsb);
+    tl->max_nsa = std::max(tl->max_nsa, G.nsa);
+    tl->max_ncob = std::max(tl->max_ncob, G.ncob);
   }
   long long cart = 0;
   for (auto &G : groups) {
@@ -2096,10 +2104,53 @@ int decontract_chunk(const TaskListImpl *tl) {
   return std::max(1, std::min(std::max(tl->max_ncoa, 1), 1024 / nsb));
 }
 
+// LDS plan of the staged (de)contraction kernels: nc columns of the
+// Cartesian block per pass, doubles per wave (decontract, contract), waves
+// per workgroup and resident workgroups; ok = false when even one column
+// does not fit (the unstaged kernels take those lists).
+struct StagePlan {
+  bool ok = false;
+  int nc_d = 1, wd_d = 0, nc_c = 1, wd_c = 0, nwaves = 4, blocks_d = 1, blocks_c = 1;
+};
+
+StagePlan stage_plan(const TaskListImpl *tl) {
+  StagePlan p;
+  const long long A = std::max(tl->max_nsa, 1), B = std::max(tl->max_nsb, 1);
+  const long long CA = std::max(tl->max_ncoa, 1), CB = std::max(tl->max_ncob, 1);
+  const long long budget = 2048;  // doubles per wave (16 KiB)
+  auto fit = [&](long long fixed, long long per, int &nc, int &wd) {
+    const long long n = std::max<long long>(1, std::min(CA, (budget - fixed) / per));
+    nc = (int)n;
+    wd = (int)((fixed + n * per + 1) & ~1LL);
+  };
+  fit(A * B + B * CB, A + B, p.nc_d, p.wd_d);
+  fit(B * CB + A * B, CB + B + A, p.nc_c, p.wd_c);
+  const long long wd = std::max(p.wd_d, p.wd_c);
+  const long long limit = lds_limit();
+  if (8 * wd > limit) return p;
+  p.nwaves = (int)std::max<long long>(1, std::min<long long>(4, 65536 / (8 * wd)));
+  auto resident = [&](int wdk) {
+    const long long per_wg = 8LL * wdk * p.nwaves;
+    return (int)std::max<long long>(1, std::min<long long>(32 / p.nwaves, limit / per_wg));
+  };
+  p.blocks_d = num_cus() * resident(p.wd_d);
+  p.blocks_c = num_cus() * resident(p.wd_c);
+  p.ok = true;
+  return p;
+}
+
 void run_decontract(TaskListImpl *tl, const double *pab, hipStream_t s,
                     double *cart = nullptr) {
-  const int cach = decontract_chunk(tl);
   TimedLaunch tm(2, s);
+  const StagePlan sp = stage_plan(tl);
+  if (sp.ok) {
+    HIP_CHECK(launch_decontract_staged((const Group *)tl->d_groups.p, tl->ngroups, pab,
+                                       (const double *)tl->d_sphi.p,
+                                       cart ? cart : (double *)tl->d_cart.p, tl->cart_len,
+                                       sp.nc_d, sp.wd_d, sp.nwaves, sp.blocks_d, s));
+    return;
+  }
+  const int cach = decontract_chunk(tl);
   HIP_CHECK(launch_decontract((const Group *)tl->d_groups.p, tl->ngroups, pab, 0,
                               (const double *)tl->d_sphi.p, cart ? cart : (double *)tl->d_cart.p,
                               tl->cart_len, cach, std::max(tl->max_nsb, 1), s));
@@ -2203,11 +2254,19 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
   }
   {
     TimedLaunch tm(3, s);
-    const int cach = decontract_chunk(tl);
-    HIP_CHECK(launch_contract((const Group *)tl->d_groups.p, tl->ngroups,
-                              (const double *)tl->d_hab_cart.p, (const double *)tl->d_sphi.p,
-                              hab, hab_len, cach, std::max(tl->max_nsb, 1),
-                              tl->max_nsa_nsb, s));
+    const StagePlan sp = stage_plan(tl);
+    if (sp.ok) {
+      HIP_CHECK(launch_contract_staged((const Group *)tl->d_groups.p, tl->ngroups,
+                                       (const double *)tl->d_hab_cart.p,
+                                       (const double *)tl->d_sphi.p, hab, hab_len, sp.nc_c,
+                                       sp.wd_c, sp.nwaves, sp.blocks_c, s));
+    } else {
+      const int cach = decontract_chunk(tl);
+      HIP_CHECK(launch_contract((const Group *)tl->d_groups.p, tl->ngroups,
+                                (const double *)tl->d_hab_cart.p, (const double *)tl->d_sphi.p,
+                                hab, hab_len, cach, std::max(tl->max_nsb, 1),
+                                tl->max_nsa_nsb, s));
+    }
   }
   if (do_f || do_v) {
     TimedLaunch tm(5, s);

@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "grid_internal.h"
 
 #include "grid_device.h"
@@ -835,6 +837,133 @@ __global__ __launch_bounds__(64 * kGroupWaves) void contract_kernel(
   }
 }
 
+// Staged (de)contraction: persistent waves, each taking groups
+// g = wave, wave + total waves, ...; per group every operand is first copied
+// into the wave's LDS with all loads in flight at once (the block's
+// (sa, sb) sub-block P, the rows sb of sphi_b, a chunk of nc columns of the
+// rows sa of sphi_a), then both products run from LDS:
+//   decontract  W[sb][c] = sum_sa P[sa][sb] Sa[sa][c],
+//               cart[cb][c] = sum_sb Sb[sb][cb] W[sb][c]
+//   contract    W[sb][c] = sum_cb Sb[sb][cb] H[cb][c]   (H: the hab_cart chunk)
+//               acc[sb][sa] += sum_c W[sb][c] Sa[sa][c], hab = acc
+// LDS per wave (doubles): decontract nsa nsb + nsb ncob + nc (nsa + nsb);
+// contract nsb ncob + nc (ncob + nsb + nsa) + nsa nsb.
+__global__ __launch_bounds__(256) void decontract_staged_kernel(
+    const Group *__restrict__ groups, int ngroups, const double *__restrict__ pab,
+    const double *__restrict__ sphi, double *__restrict__ cart, long long cart_len, int nc_max,
+    int wave_doubles) {
+  extern __shared__ double lds_ds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  double *base = lds_ds + (size_t)wave * wave_doubles;
+  for (int gi = blockIdx.x * nw + wave; gi < ngroups; gi += gridDim.x * nw) {
+    const Group g = groups[gi];
+    const int nsa = g.nsa, nsb = g.nsb, ncoa = g.ncoa, ncob = g.ncob;
+    double *P = base;                   // [nsa][nsb]
+    double *Sb = P + nsa * nsb;         // [nsb][ncob]
+    double *Sa = Sb + nsb * ncob;       // [nsa][nc]
+    double *W = Sa + nsa * nc_max;      // [nsb][nc]
+    const double *blk = pab + g.block_off;
+    if (g.transpose) {
+      for (int e = lane; e < nsa * nsb; e += 64) {
+        const int sb = e / nsa, sa = e - sb * nsa;
+        P[sa * nsb + sb] = blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa];
+      }
+    } else {
+      for (int e = lane; e < nsa * nsb; e += 64) {
+        const int sa = e / nsb, sb = e - sa * nsb;
+        P[e] = blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
+      }
+    }
+    for (int e = lane; e < nsb * ncob; e += 64) {
+      const int sb = e / ncob, cb = e - sb * ncob;
+      Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+    }
+    for (int c0 = 0; c0 < ncoa; c0 += nc_max) {
+      const int nc = min(nc_max, ncoa - c0);
+      for (int e = lane; e < nsa * nc; e += 64) {
+        const int sa = e / nc, c = e - sa * nc;
+        Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
+      }
+      wave_sync();
+      for (int e = lane; e < nsb * nc; e += 64) {
+        const int sb = e / nc, c = e - sb * nc;
+        double acc = 0.0;
+        for (int sa = 0; sa < nsa; sa++) acc = fma(P[sa * nsb + sb], Sa[sa * nc + c], acc);
+        W[e] = acc;
+      }
+      wave_sync();
+      for (int e = lane; e < ncob * nc; e += 64) {
+        const int cb = e / nc, c = e - cb * nc;
+        double acc = 0.0;
+        for (int sb = 0; sb < nsb; sb++) acc = fma(Sb[sb * ncob + cb], W[sb * nc + c], acc);
+        const long long q = g.cart_off + (long long)cb * ncoa + c0 + c;
+        if (q < cart_len) cart[q] = acc;
+      }
+      wave_sync();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void contract_staged_kernel(
+    const Group *__restrict__ groups, int ngroups, const double *__restrict__ hab_cart,
+    const double *__restrict__ sphi, double *__restrict__ hab, long long hab_len, int nc_max,
+    int wave_doubles) {
+  extern __shared__ double lds_cs[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  double *base = lds_cs + (size_t)wave * wave_doubles;
+  for (int gi = blockIdx.x * nw + wave; gi < ngroups; gi += gridDim.x * nw) {
+    const Group g = groups[gi];
+    const int nsa = g.nsa, nsb = g.nsb, ncoa = g.ncoa, ncob = g.ncob;
+    double *acc = base;                 // [nsb][nsa]
+    double *Sb = acc + nsa * nsb;       // [nsb][ncob]
+    double *H = Sb + nsb * ncob;        // [ncob][nc]
+    double *W = H + ncob * nc_max;      // [nsb][nc]
+    double *Sa = W + nsb * nc_max;      // [nsa][nc]
+    for (int e = lane; e < nsa * nsb; e += 64) acc[e] = 0.0;
+    for (int e = lane; e < nsb * ncob; e += 64) {
+      const int sb = e / ncob, cb = e - sb * ncob;
+      Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+    }
+    for (int c0 = 0; c0 < ncoa; c0 += nc_max) {
+      const int nc = min(nc_max, ncoa - c0);
+      for (int e = lane; e < ncob * nc; e += 64) {
+        const int cb = e / nc, c = e - cb * nc;
+        H[e] = hab_cart[g.cart_off + (long long)cb * ncoa + c0 + c];
+      }
+      for (int e = lane; e < nsa * nc; e += 64) {
+        const int sa = e / nc, c = e - sa * nc;
+        Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
+      }
+      wave_sync();
+      for (int e = lane; e < nsb * nc; e += 64) {
+        const int sb = e / nc, c = e - sb * nc;
+        double s = 0.0;
+        for (int cb = 0; cb < ncob; cb++) s = fma(Sb[sb * ncob + cb], H[cb * nc + c], s);
+        W[e] = s;
+      }
+      wave_sync();
+      for (int e = lane; e < nsa * nsb; e += 64) {
+        const int sb = e / nsa, sa = e - sb * nsa;
+        double s = 0.0;
+        for (int c = 0; c < nc; c++) s = fma(W[sb * nc + c], Sa[sa * nc + c], s);
+        acc[e] += s;
+      }
+      wave_sync();
+    }
+    for (int e = lane; e < nsa * nsb; e += 64) {
+      const int sb = e / nsa, sa = e - sb * nsa;
+      const long long q = g.block_off + (g.transpose ? (long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa
+                                                     : (long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb);
+      if (q < hab_len) hab[q] = acc[e];
+    }
+    wave_sync();
+  }
+}
+
 // FP32-mixed integrate: the potential rounded to FP32 once per call.
 __global__ void to_float_kernel(const double *src, float *dst, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -957,6 +1086,34 @@ hipError_t launch_integrate(const PointParams &P, bool ortho, hipStream_t s) {
                 : launch_integrate_kind<false, true>(P, lp, lds, s);
   return full ? launch_integrate_kind<true, false>(P, lp, lds, s)
               : launch_integrate_kind<false, false>(P, lp, lds, s);
+}
+
+// Staged kernels: nwaves waves per workgroup of wave_doubles each, at most
+// `blocks` workgroups (persistent: the waves loop over the groups).
+hipError_t launch_decontract_staged(const Group *groups, int ngroups, const double *pab,
+                                    const double *sphi, double *cart, long long cart_len, int nc,
+                                    int wave_doubles, int nwaves, int blocks, hipStream_t s) {
+  if (ngroups == 0) return hipSuccess;
+  const size_t lds = sizeof(double) * (size_t)wave_doubles * nwaves;
+  const hipError_t e = allow_lds(decontract_staged_kernel, lds);
+  if (e != hipSuccess) return e;
+  blocks = std::max(1, std::min(blocks, (ngroups + nwaves - 1) / nwaves));
+  hipLaunchKernelGGL(decontract_staged_kernel, dim3(blocks), dim3(64 * nwaves), lds, s, groups,
+                     ngroups, pab, sphi, cart, cart_len, nc, wave_doubles);
+  return hipGetLastError();
+}
+
+hipError_t launch_contract_staged(const Group *groups, int ngroups, const double *hab_cart,
+                                  const double *sphi, double *hab, long long hab_len, int nc,
+                                  int wave_doubles, int nwaves, int blocks, hipStream_t s) {
+  if (ngroups == 0) return hipSuccess;
+  const size_t lds = sizeof(double) * (size_t)wave_doubles * nwaves;
+  const hipError_t e = allow_lds(contract_staged_kernel, lds);
+  if (e != hipSuccess) return e;
+  blocks = std::max(1, std::min(blocks, (ngroups + nwaves - 1) / nwaves));
+  hipLaunchKernelGGL(contract_staged_kernel, dim3(blocks), dim3(64 * nwaves), lds, s, groups,
+                     ngroups, hab_cart, sphi, hab, hab_len, nc, wave_doubles);
+  return hipGetLastError();
 }
 
 hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,
