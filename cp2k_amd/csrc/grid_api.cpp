@@ -59,6 +59,8 @@ hipError_t launch_collocate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_collocate_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 hipError_t launch_integrate_wave(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s);
+bool finish_lane_ok(const PointParams &P, int lp);
+hipError_t launch_finish_lane(const PointParams &P, hipStream_t s);
 hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s);
 hipError_t launch_finish_block(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
@@ -1549,6 +1551,10 @@ void launch_ortho(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
           PointParams F = Q;
           F.task_base = a;
           F.ntasks = b - a;
+          if (finish_lane_ok(F, v)) {
+            HIP_CHECK(launch_finish_lane(F, s));
+            continue;
+          }
           F.wp = make_wave_plan(tl, l, v, P.ld, kPlanFinish);
           HIP_CHECK(launch_finish_wave(F, s));
         }
@@ -1751,6 +1757,10 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
           PointParams F = Q;
           F.task_base = a;
           F.ntasks = b - a;
+          if (finish_lane_ok(F, v)) {
+            HIP_CHECK(launch_finish_lane(F, s));
+            continue;
+          }
           F.wp = make_wave_plan(tl, l, v, P.ld, kPlanFinish);
           HIP_CHECK(launch_finish_wave(F, s));
         }

@@ -702,6 +702,165 @@ __device__ __forceinline__ void lane_cxyz(const PointParams &P, const Task &T, c
 }
 
 // ---------------------------------------------------------------------------
+// Lane-per-task cxyz -> cab -> hab (and forces) for small l: the adjoint of
+// lane_cxyz.  LA, LB: the shifted l of the call (forces raise la by one and
+// lower both l_min by one, src/grid/ref/grid_ref_integrate.c); cab[A][B] for
+// every orbital pair below LA, LB is a compile-time unrolled sum
+//   cab(A, B) = pexp sum_{lx,ly,lz} ax[lx] ay[ly] az[lz] cxyz[lx, ly, lz]
+// with the per-axis expansions of lane_cxyz; hab and the force getters of
+// grid_process_vab.h:29-205 then read the needed entries.  Each lane adds
+// its task's hab sub-block with global atomics (images of one atom pair
+// share it) and writes its own forces (no cross-lane reduction).
+// ---------------------------------------------------------------------------
+constexpr int cidx_c(int x, int y, int z) {
+  return x + y + z == 0 ? 0
+                        : ncoset_c(x + y + z - 1) + ((y + z) * (y + z + 1)) / 2 + z;
+}
+
+template <int LA, int LB, int IA, int IB>
+__device__ __forceinline__ double lane_cab(const double (&pa)[3][LA + 1],
+                                           const double (&pb)[3][LB + 1], const double *cx) {
+  constexpr int LP = LA + LB;
+  constexpr int AX = kOrb.v[IA][0], AY = kOrb.v[IA][1], AZ = kOrb.v[IA][2];
+  constexpr int BX = kOrb.v[IB][0], BY = kOrb.v[IB][1], BZ = kOrb.v[IB][2];
+  double ax[AX + BX + 1], ay[AY + BY + 1], az[AZ + BZ + 1];
+  axis_alpha<AX, BX>(pa[0], pb[0], ax);
+  axis_alpha<AY, BY>(pa[1], pb[1], ay);
+  axis_alpha<AZ, BZ>(pa[2], pb[2], az);
+  double s = 0.0;
+#pragma unroll
+  for (int lz = 0; lz <= AZ + BZ; lz++) {
+    double u = 0.0;
+#pragma unroll
+    for (int ly = 0; ly <= AY + BY; ly++) {
+      double t = 0.0;
+#pragma unroll
+      for (int lx = 0; lx <= AX + BX; lx++) t = fma(ax[lx], cx[pidx_c(lx, ly, lz, LP)], t);
+      u = fma(ay[ly], t, u);
+    }
+    s = fma(az[lz], u, s);
+  }
+  return s;
+}
+
+template <int LA, int LB, int IA, int IB>
+struct CabFill {
+  static __device__ __forceinline__ void run(const double (&pa)[3][LA + 1],
+                                             const double (&pb)[3][LB + 1], const double *cx,
+                                             double pexp, double (&cab)[ncoset_c(LA)][ncoset_c(LB)]) {
+    cab[IA][IB] = pexp * lane_cab<LA, LB, IA, IB>(pa, pb, cx);
+    if constexpr (IA + 1 < ncoset_c(LA))
+      CabFill<LA, LB, IA + 1, IB>::run(pa, pb, cx, pexp, cab);
+    else if constexpr (IB + 1 < ncoset_c(LB))
+      CabFill<LA, LB, 0, IB + 1>::run(pa, pb, cx, pexp, cab);
+  }
+};
+
+// hab and forces of orbital pair (IA, IB) (l below the task's la_max, lb_max).
+template <int LA, int LB, int IA, int IB, bool FORCES>
+struct GetLoop {
+  static __device__ __forceinline__ void run(const PointParams &P, const Task &T, const Group &G,
+                                             const double (&cab)[ncoset_c(LA)][ncoset_c(LB)],
+                                             int na, int nb, double *f) {
+    constexpr int AX = kOrb.v[IA][0], AY = kOrb.v[IA][1], AZ = kOrb.v[IA][2];
+    constexpr int BX = kOrb.v[IB][0], BY = kOrb.v[IB][1], BZ = kOrb.v[IB][2];
+    if (AX + AY + AZ >= T.la_min && BX + BY + BZ >= T.lb_min) {
+      const long long q = G.cart_off + (long long)(T.o2 + IB) * G.ncoa + T.o1 + IA;
+      if (q < P.cart_len) {
+        atomicAdd(&P.hab_cart[q], cab[IA][IB]);
+        if constexpr (FORCES) {
+          const double pv = P.cart[q];
+          constexpr int A[3] = {AX, AY, AZ}, B[3] = {BX, BY, BZ};
+          constexpr int up[3] = {cidx_c(AX + 1, AY, AZ), cidx_c(AX, AY + 1, AZ),
+                                 cidx_c(AX, AY, AZ + 1)};
+          constexpr int dna[3] = {AX > 0 ? cidx_c(AX - 1, AY, AZ) : 0,
+                                  AY > 0 ? cidx_c(AX, AY - 1, AZ) : 0,
+                                  AZ > 0 ? cidx_c(AX, AY, AZ - 1) : 0};
+          constexpr int dnb[3] = {BX > 0 ? cidx_c(BX - 1, BY, BZ) : 0,
+                                  BY > 0 ? cidx_c(BX, BY - 1, BZ) : 0,
+                                  BZ > 0 ? cidx_c(BX, BY, BZ - 1) : 0};
+          const double c0 = cab[IA][IB];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            const double cu = cab[up[i]][IB];
+            const double cd = A[i] > 0 ? (double)A[i] * cab[dna[i]][IB] : 0.0;
+            const double cb = B[i] > 0 ? (double)B[i] * cab[IA][dnb[i]] : 0.0;
+            f[i] += pv * (2.0 * T.zeta * cu - cd);
+            f[3 + i] += pv * (2.0 * T.zetb * (cu - T.rab[i] * c0) - cb);
+          }
+        }
+      }
+    }
+    constexpr int LA0 = FORCES ? LA - 1 : LA;
+    if constexpr (IA + 1 < ncoset_c(LA0))
+      GetLoop<LA, LB, IA + 1, IB, FORCES>::run(P, T, G, cab, na, nb, f);
+    else if constexpr (IB + 1 < ncoset_c(LB))
+      GetLoop<LA, LB, 0, IB + 1, FORCES>::run(P, T, G, cab, na, nb, f);
+  }
+};
+
+template <int LA0, int LB0, bool FORCES>
+__device__ __forceinline__ void lane_finish(const PointParams &P, const Task &T, const Group &G,
+                                            long long task) {
+  constexpr int LA = LA0 + (FORCES ? 1 : 0), LB = LB0;
+  constexpr int LP = LA + LB;
+  constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
+  double cx[NC];
+  const double *in = P.cxyz_all + task * P.cxyz_stride;
+#pragma unroll
+  for (int e = 0; e < NC; e++) cx[e] = in[e];
+  double pa[3][LA + 1], pb[3][LB + 1];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double da = T.rp[d] - T.ra[d], db = T.rp[d] - T.rb[d];
+    pa[d][0] = 1.0;
+    pb[d][0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= LA; k++) pa[d][k] = pa[d][k - 1] * da;
+#pragma unroll
+    for (int k = 1; k <= LB; k++) pb[d][k] = pb[d][k - 1] * db;
+  }
+  double cab[ncoset_c(LA)][ncoset_c(LB)];
+  CabFill<LA, LB, 0, 0>::run(pa, pb, cx, T.pexp, cab);
+  double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  GetLoop<LA, LB, 0, 0, FORCES>::run(P, T, G, cab, 0, 0, f);
+  if constexpr (FORCES) {
+    double *out = P.task_out + task * kTaskOutStride;
+    const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) out[c] = scale * f[c];
+#pragma unroll
+    for (int c = 6; c < kTaskOutStride; c++) out[c] = 0.0;
+  }
+}
+
+// A task per lane (la_max + lb_max <= 2 with forces, <= 3 without; no
+// virial, no tau: the host routes only those here).
+template <bool FORCES>
+__global__ __launch_bounds__(256) void finish_lane_kernel(const Task *__restrict__ tasks,
+                                                          const Group *__restrict__ groups,
+                                                          PointParams P) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.ntasks) return;
+  const long long task = launch_task(P, t);
+  const Task &T = tasks[task];
+  const Group &G = groups[T.group];
+  switch (T.la_max * 4 + T.lb_max) {
+    case 0: lane_finish<0, 0, FORCES>(P, T, G, task); break;
+    case 1: lane_finish<0, 1, FORCES>(P, T, G, task); break;
+    case 2: lane_finish<0, 2, FORCES>(P, T, G, task); break;
+    case 4: lane_finish<1, 0, FORCES>(P, T, G, task); break;
+    case 5: lane_finish<1, 1, FORCES>(P, T, G, task); break;
+    case 8: lane_finish<2, 0, FORCES>(P, T, G, task); break;
+    case 3: if constexpr (!FORCES) lane_finish<0, 3, FORCES>(P, T, G, task); break;
+    case 6: if constexpr (!FORCES) lane_finish<1, 2, FORCES>(P, T, G, task); break;
+    case 9: if constexpr (!FORCES) lane_finish<2, 1, FORCES>(P, T, G, task); break;
+    case 12: if constexpr (!FORCES) lane_finish<3, 0, FORCES>(P, T, G, task); break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Integrate for lp 7..11 on the matrix cores.  Per task the coefficients
 //   cxyz[lx][(ly, lz)] = sum over columns c of px[c][lx] * (py[c][ly] M[c][lz])
 // (M = the column's z moments) are a (lp+1) x NQ x ncol product; the
@@ -1248,6 +1407,24 @@ hipError_t launch_finish_block(const PointParams &P, hipStream_t s) {
   };
   if (P.compute_tau) return go(finish_block_kernel<2>);
   return full ? go(finish_block_kernel<1>) : go(finish_block_kernel<0>);
+}
+
+// The lane kernel serves calls without virial and tau, lp <= 3 after the
+// shift (la_max + lb_max <= 2 with forces, <= 3 without); false: use the
+// wave kernel.
+bool finish_lane_ok(const PointParams &P, int lp) {
+  return !P.do_virial && !P.compute_tau && lp <= 3 && P.ld[2] == 0 &&
+         P.ld[0] == (P.do_forces ? 1 : 0);
+}
+
+hipError_t launch_finish_lane(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((P.ntasks + 255) / 256);
+  if (P.do_forces)
+    hipLaunchKernelGGL(finish_lane_kernel<true>, dim3(blocks), dim3(256), 0, s, P.tasks, P.groups, P);
+  else
+    hipLaunchKernelGGL(finish_lane_kernel<false>, dim3(blocks), dim3(256), 0, s, P.tasks, P.groups, P);
+  return hipGetLastError();
 }
 
 hipError_t launch_finish_wave(const PointParams &P, hipStream_t s) {
