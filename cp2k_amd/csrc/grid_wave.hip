@@ -743,59 +743,56 @@ __device__ __forceinline__ double lane_cab(const double (&pa)[3][LA + 1],
   return s;
 }
 
-template <int LA, int LB, int IA, int IB>
-struct CabFill {
-  static __device__ __forceinline__ void run(const double (&pa)[3][LA + 1],
-                                             const double (&pb)[3][LB + 1], const double *cx,
-                                             double pexp, double (&cab)[ncoset_c(LA)][ncoset_c(LB)]) {
-    cab[IA][IB] = pexp * lane_cab<LA, LB, IA, IB>(pa, pb, cx);
-    if constexpr (IA + 1 < ncoset_c(LA))
-      CabFill<LA, LB, IA + 1, IB>::run(pa, pb, cx, pexp, cab);
-    else if constexpr (IB + 1 < ncoset_c(LB))
-      CabFill<LA, LB, 0, IB + 1>::run(pa, pb, cx, pexp, cab);
-  }
+// The fields of a task and its group that the lane finish reads, copied to
+// registers once (the hab atomics could otherwise alias them, and every
+// scattered reload costs the address unit a cache line per lane).
+struct LaneTask {
+  long long base;   // cart offset of the task's (o1, o2) corner
+  int ncoa, la_min, lb_min;
+  double pexp, zeta, zetb, rab[3];
 };
 
-// hab and forces of orbital pair (IA, IB) (l below the task's la_max, lb_max).
+// hab and forces of orbital pair (IA, IB) (l up to the task's la_max,
+// lb_max), each cab entry evaluated where it is used.
 template <int LA, int LB, int IA, int IB, bool FORCES>
 struct GetLoop {
-  static __device__ __forceinline__ void run(const PointParams &P, const Task &T, const Group &G,
-                                             const double (&cab)[ncoset_c(LA)][ncoset_c(LB)],
-                                             int na, int nb, double *f) {
+  static __device__ __forceinline__ void run(const PointParams &P, const LaneTask &L,
+                                             const double (&pa)[3][LA + 1],
+                                             const double (&pb)[3][LB + 1], const double *cx,
+                                             double *f) {
     constexpr int AX = kOrb.v[IA][0], AY = kOrb.v[IA][1], AZ = kOrb.v[IA][2];
     constexpr int BX = kOrb.v[IB][0], BY = kOrb.v[IB][1], BZ = kOrb.v[IB][2];
-    if (AX + AY + AZ >= T.la_min && BX + BY + BZ >= T.lb_min) {
-      const long long q = G.cart_off + (long long)(T.o2 + IB) * G.ncoa + T.o1 + IA;
+    if (AX + AY + AZ >= L.la_min && BX + BY + BZ >= L.lb_min) {
+      const long long q = L.base + (long long)IB * L.ncoa + IA;
       if (q < P.cart_len) {
-        atomicAdd(&P.hab_cart[q], cab[IA][IB]);
+        const double c0 = L.pexp * lane_cab<LA, LB, IA, IB>(pa, pb, cx);
+        __hip_atomic_fetch_add(&P.hab_cart[q], c0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if constexpr (FORCES) {
-          const double pv = P.cart[q];
-          constexpr int A[3] = {AX, AY, AZ}, B[3] = {BX, BY, BZ};
-          constexpr int up[3] = {cidx_c(AX + 1, AY, AZ), cidx_c(AX, AY + 1, AZ),
-                                 cidx_c(AX, AY, AZ + 1)};
-          constexpr int dna[3] = {AX > 0 ? cidx_c(AX - 1, AY, AZ) : 0,
-                                  AY > 0 ? cidx_c(AX, AY - 1, AZ) : 0,
-                                  AZ > 0 ? cidx_c(AX, AY, AZ - 1) : 0};
-          constexpr int dnb[3] = {BX > 0 ? cidx_c(BX - 1, BY, BZ) : 0,
-                                  BY > 0 ? cidx_c(BX, BY - 1, BZ) : 0,
-                                  BZ > 0 ? cidx_c(BX, BY, BZ - 1) : 0};
-          const double c0 = cab[IA][IB];
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            const double cu = cab[up[i]][IB];
-            const double cd = A[i] > 0 ? (double)A[i] * cab[dna[i]][IB] : 0.0;
-            const double cb = B[i] > 0 ? (double)B[i] * cab[IA][dnb[i]] : 0.0;
-            f[i] += pv * (2.0 * T.zeta * cu - cd);
-            f[3 + i] += pv * (2.0 * T.zetb * (cu - T.rab[i] * c0) - cb);
-          }
+          const double pv = __builtin_nontemporal_load(&P.cart[q]);
+          const double u0 = L.pexp * lane_cab<LA, LB, cidx_c(AX + 1, AY, AZ), IB>(pa, pb, cx);
+          const double u1 = L.pexp * lane_cab<LA, LB, cidx_c(AX, AY + 1, AZ), IB>(pa, pb, cx);
+          const double u2 = L.pexp * lane_cab<LA, LB, cidx_c(AX, AY, AZ + 1), IB>(pa, pb, cx);
+          double d0 = 0.0, d1 = 0.0, d2 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+          if constexpr (AX > 0) d0 = AX * L.pexp * lane_cab<LA, LB, cidx_c(AX - 1, AY, AZ), IB>(pa, pb, cx);
+          if constexpr (AY > 0) d1 = AY * L.pexp * lane_cab<LA, LB, cidx_c(AX, AY - 1, AZ), IB>(pa, pb, cx);
+          if constexpr (AZ > 0) d2 = AZ * L.pexp * lane_cab<LA, LB, cidx_c(AX, AY, AZ - 1), IB>(pa, pb, cx);
+          if constexpr (BX > 0) e0 = BX * L.pexp * lane_cab<LA, LB, IA, cidx_c(BX - 1, BY, BZ)>(pa, pb, cx);
+          if constexpr (BY > 0) e1 = BY * L.pexp * lane_cab<LA, LB, IA, cidx_c(BX, BY - 1, BZ)>(pa, pb, cx);
+          if constexpr (BZ > 0) e2 = BZ * L.pexp * lane_cab<LA, LB, IA, cidx_c(BX, BY, BZ - 1)>(pa, pb, cx);
+          f[0] += pv * (2.0 * L.zeta * u0 - d0);
+          f[1] += pv * (2.0 * L.zeta * u1 - d1);
+          f[2] += pv * (2.0 * L.zeta * u2 - d2);
+          f[3] += pv * (2.0 * L.zetb * (u0 - L.rab[0] * c0) - e0);
+          f[4] += pv * (2.0 * L.zetb * (u1 - L.rab[1] * c0) - e1);
+          f[5] += pv * (2.0 * L.zetb * (u2 - L.rab[2] * c0) - e2);
         }
       }
     }
     constexpr int LA0 = FORCES ? LA - 1 : LA;
     if constexpr (IA + 1 < ncoset_c(LA0))
-      GetLoop<LA, LB, IA + 1, IB, FORCES>::run(P, T, G, cab, na, nb, f);
+      GetLoop<LA, LB, IA + 1, IB, FORCES>::run(P, L, pa, pb, cx, f);
     else if constexpr (IB + 1 < ncoset_c(LB))
-      GetLoop<LA, LB, 0, IB + 1, FORCES>::run(P, T, G, cab, na, nb, f);
+      GetLoop<LA, LB, 0, IB + 1, FORCES>::run(P, L, pa, pb, cx, f);
   }
 };
 
@@ -805,13 +802,18 @@ __device__ __forceinline__ void lane_finish(const PointParams &P, const Task &T,
   constexpr int LA = LA0 + (FORCES ? 1 : 0), LB = LB0;
   constexpr int LP = LA + LB;
   constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
-  double cx[NC];
-  const double *in = P.cxyz_all + task * P.cxyz_stride;
-#pragma unroll
-  for (int e = 0; e < NC; e++) cx[e] = in[e];
+  LaneTask L;
+  L.ncoa = G.ncoa;
+  L.base = G.cart_off + (long long)T.o2 * L.ncoa + T.o1;
+  L.la_min = T.la_min;
+  L.lb_min = T.lb_min;
+  L.pexp = T.pexp;
+  L.zeta = T.zeta;
+  L.zetb = T.zetb;
   double pa[3][LA + 1], pb[3][LB + 1];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
+    L.rab[d] = T.rab[d];
     const double da = T.rp[d] - T.ra[d], db = T.rp[d] - T.rb[d];
     pa[d][0] = 1.0;
     pb[d][0] = 1.0;
@@ -820,13 +822,15 @@ __device__ __forceinline__ void lane_finish(const PointParams &P, const Task &T,
 #pragma unroll
     for (int k = 1; k <= LB; k++) pb[d][k] = pb[d][k - 1] * db;
   }
-  double cab[ncoset_c(LA)][ncoset_c(LB)];
-  CabFill<LA, LB, 0, 0>::run(pa, pb, cx, T.pexp, cab);
+  const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
+  double cx[NC];
+  const double *in = P.cxyz_all + task * P.cxyz_stride;
+#pragma unroll
+  for (int e = 0; e < NC; e++) cx[e] = in[e];
   double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  GetLoop<LA, LB, 0, 0, FORCES>::run(P, T, G, cab, 0, 0, f);
+  GetLoop<LA, LB, 0, 0, FORCES>::run(P, L, pa, pb, cx, f);
   if constexpr (FORCES) {
     double *out = P.task_out + task * kTaskOutStride;
-    const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
 #pragma unroll
     for (int c = 0; c < 6; c++) out[c] = scale * f[c];
 #pragma unroll
@@ -834,8 +838,8 @@ __device__ __forceinline__ void lane_finish(const PointParams &P, const Task &T,
   }
 }
 
-// A task per lane (la_max + lb_max <= 2 with forces, <= 3 without; no
-// virial, no tau: the host routes only those here).
+// A task per lane (la_max + lb_max <= 3, no virial, no tau: the host
+// routes only those here).
 template <bool FORCES>
 __global__ __launch_bounds__(256) void finish_lane_kernel(const Task *__restrict__ tasks,
                                                           const Group *__restrict__ groups,
@@ -852,10 +856,10 @@ __global__ __launch_bounds__(256) void finish_lane_kernel(const Task *__restrict
     case 4: lane_finish<1, 0, FORCES>(P, T, G, task); break;
     case 5: lane_finish<1, 1, FORCES>(P, T, G, task); break;
     case 8: lane_finish<2, 0, FORCES>(P, T, G, task); break;
-    case 3: if constexpr (!FORCES) lane_finish<0, 3, FORCES>(P, T, G, task); break;
-    case 6: if constexpr (!FORCES) lane_finish<1, 2, FORCES>(P, T, G, task); break;
-    case 9: if constexpr (!FORCES) lane_finish<2, 1, FORCES>(P, T, G, task); break;
-    case 12: if constexpr (!FORCES) lane_finish<3, 0, FORCES>(P, T, G, task); break;
+    case 3: lane_finish<0, 3, FORCES>(P, T, G, task); break;
+    case 6: lane_finish<1, 2, FORCES>(P, T, G, task); break;
+    case 9: lane_finish<2, 1, FORCES>(P, T, G, task); break;
+    case 12: lane_finish<3, 0, FORCES>(P, T, G, task); break;
     default: break;
   }
 }
@@ -1409,11 +1413,11 @@ hipError_t launch_finish_block(const PointParams &P, hipStream_t s) {
   return full ? go(finish_block_kernel<1>) : go(finish_block_kernel<0>);
 }
 
-// The lane kernel serves calls without virial and tau, lp <= 3 after the
-// shift (la_max + lb_max <= 2 with forces, <= 3 without); false: use the
-// wave kernel.
+// The lane kernel serves calls without virial and tau for la_max + lb_max
+// <= 3 (lp: after the call's shift); false: use the wave kernel.
 bool finish_lane_ok(const PointParams &P, int lp) {
-  return !P.do_virial && !P.compute_tau && lp <= 3 && P.ld[2] == 0 &&
+  const int lp0 = lp - P.ld[0] - P.ld[2];
+  return !P.do_virial && !P.compute_tau && lp0 <= 3 && P.ld[2] == 0 &&
          P.ld[0] == (P.do_forces ? 1 : 0);
 }
 
