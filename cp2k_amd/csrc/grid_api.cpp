@@ -476,7 +476,7 @@ struct TaskListImpl {
   const Level *static_levels = nullptr;    // fixed level table (graph capture)
   ListInputs inputs;                       // the create arguments (for batching)
   std::vector<signed char> lp0s;  // lp0 per sorted task
-  std::vector<int> highl_before;  // sorted tasks before s with la_max or lb_max > 2
+  std::vector<int> highl_before;  // sorted tasks before s outside lane_cxyz_task
   int ngeo = 0;
   long long class_points = 0;     // task points of the tasks with a geometry class
   std::vector<long long> class_npts;  // per sorted task: points of its class (or 0)
@@ -978,7 +978,8 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
   tl->highl_before.assign(ntasks + 1, 0);
   for (int s = 0; s < ntasks; s++)
     tl->highl_before[s + 1] =
-        tl->highl_before[s] + (tasks[s].la_max > 2 || tasks[s].lb_max > 2 ? 1 : 0);
+        tl->highl_before[s] +
+        (lane_cxyz_task(tasks[s].la_max, tasks[s].lb_max) ? 0 : 1);
   for (int s = 0; s < ntasks; s++) {
     const Task &T = tasks[s];
     if (!(T.flags & kTaskOrtho)) continue;

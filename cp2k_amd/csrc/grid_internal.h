@@ -116,6 +116,12 @@ struct LdsPlan {
 // Slots per level of the general change-of-variables matrices (lp 0..11).
 constexpr int kGmatSlots = 12;
 
+// Tasks of the lane-per-task cab -> cxyz kernel: la_max, lb_max <= 3 with
+// la_max + lb_max <= 4 (larger pairs unroll into too much code).
+__host__ __device__ constexpr bool lane_cxyz_task(int la, int lb) {
+  return la <= 3 && lb <= 3 && la + lb <= 4;
+}
+
 // Register-array bound of the point kernels for a maximum lp.
 inline int lpb_bucket(int lp) { return lp <= 4 ? 4 : lp <= 8 ? 8 : lp <= 12 ? 12 : 16; }
 

@@ -601,7 +601,8 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
 
 // ---------------------------------------------------------------------------
 // Lane-per-task cab -> cxyz for GRID_FUNC_AB (cab = the density sub-block),
-// la, lb <= 2: every loop over orbitals, binomials and (lx, ly, lz) is
+// la, lb <= 3 with la + lb <= 4 (lane_cxyz_task): every loop over orbitals,
+// binomials and (lx, ly, lz) is
 // unrolled at compile time; accumulators live in registers.
 // (x - a)^ax (x - b)^bx = sum_l alpha[l] (x - p)^l with
 // alpha[k + m] += C(ax, k) C(bx, m) (p - a)^(ax - k) (p - b)^(bx - m),
@@ -986,19 +987,24 @@ __global__ __launch_bounds__(256) void cxyz_lane_kernel(const Task *__restrict__
   if (t >= P.ntasks) return;
   const long long task = launch_task(P, t);
   const Task &T = tasks[task];
-  if (T.la_max > 2 || T.lb_max > 2) return;  // cxyz_wave_kernel takes these
+  if (!lane_cxyz_task(T.la_max, T.lb_max)) return;  // cxyz_wave_kernel takes these
   const Group &G = groups[T.group];
   double *out = P.cxyz_all + task * P.cxyz_stride;
-  switch (T.la_max * 3 + T.lb_max) {
+  switch (T.la_max * 4 + T.lb_max) {
     case 0: lane_cxyz<0, 0>(P, T, G, out); break;
     case 1: lane_cxyz<0, 1>(P, T, G, out); break;
     case 2: lane_cxyz<0, 2>(P, T, G, out); break;
-    case 3: lane_cxyz<1, 0>(P, T, G, out); break;
-    case 4: lane_cxyz<1, 1>(P, T, G, out); break;
-    case 5: lane_cxyz<1, 2>(P, T, G, out); break;
-    case 6: lane_cxyz<2, 0>(P, T, G, out); break;
-    case 7: lane_cxyz<2, 1>(P, T, G, out); break;
-    default: lane_cxyz<2, 2>(P, T, G, out); break;
+    case 3: lane_cxyz<0, 3>(P, T, G, out); break;
+    case 4: lane_cxyz<1, 0>(P, T, G, out); break;
+    case 5: lane_cxyz<1, 1>(P, T, G, out); break;
+    case 6: lane_cxyz<1, 2>(P, T, G, out); break;
+    case 7: lane_cxyz<1, 3>(P, T, G, out); break;
+    case 8: lane_cxyz<2, 0>(P, T, G, out); break;
+    case 9: lane_cxyz<2, 1>(P, T, G, out); break;
+    case 10: lane_cxyz<2, 2>(P, T, G, out); break;
+    case 12: lane_cxyz<3, 0>(P, T, G, out); break;
+    case 13: lane_cxyz<3, 1>(P, T, G, out); break;
+    default: break;
   }
 }
 
@@ -1014,7 +1020,7 @@ __global__ __launch_bounds__(256) void cxyz_wave_kernel(const Task *__restrict__
   if (t >= P.ntasks) return;
   const long long task = launch_task(P, t);
   const Task &T = tasks[task];
-  if (P.lane_cxyz && T.la_max <= 2 && T.lb_max <= 2) return;
+  if (P.lane_cxyz && lane_cxyz_task(T.la_max, T.lb_max)) return;
   const TaskL l = task_l(P, T);
   const WaveLds W = wave_lds(lds, P.wp, wave);
   const int NP = P.wp.np;
