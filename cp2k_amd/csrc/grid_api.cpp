@@ -65,6 +65,7 @@ hipError_t launch_integrate_mfma(const PointParams &P, hipStream_t s);
 hipError_t launch_finish_block(const PointParams &P, hipStream_t s);
 hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s);
 hipError_t launch_general_cijk(const PointParams &P, hipStream_t s);
+hipError_t launch_general_cijk_lane(const PointParams &P, hipStream_t s);
 hipError_t launch_general_integrate(const PointParams &P, int lo, int hi, hipStream_t s);
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s);
 double *noalloc_scratch(long long *doubles, hipError_t *err);
@@ -1715,7 +1716,15 @@ void launch_general(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         if (ab) HIP_CHECK(launch_cxyz_lane(Q, s));
         if (!ab || tl->highl_before[c2] > tl->highl_before[g.begin])
           HIP_CHECK(launch_cxyz_wave(Q, s));
-        HIP_CHECK(launch_general_cijk(Q, s));  // the general kernels take cijk
+        // the general kernels take cijk: lp <= 4 a task per lane, lp 5..6 a
+        // task per wave
+        const int c5 = std::min(c2, std::max(g.begin, first(5 - sh)));
+        PointParams Qa = Q, Qb = Q;
+        Qa.ntasks = c5 - g.begin;
+        Qb.task_base = c5;
+        Qb.ntasks = c2 - c5;
+        HIP_CHECK(launch_general_cijk_lane(Qa, s));
+        HIP_CHECK(launch_general_cijk(Qb, s));
       }
       if (mode == kIntegrate) {
         // one launch per lp: the lane sums take the registers that lp needs
@@ -1891,8 +1900,9 @@ void launch_merged(TaskListImpl *tl, PointParams P, hipStream_t s, int mode) {
         H.lane_cxyz = 0;
         HIP_CHECK(launch_cxyz_wave(H, s));
       }
-      PointParams G = sub(kMGenA);  // kMGenA and kMGenB are adjacent: cxyz -> cijk
-      G.ntasks = mp.off[kMGenB + 1] - mp.off[kMGenA];
+      PointParams G = sub(kMGenA);  // cxyz -> cijk: lp <= 4 a task per lane
+      HIP_CHECK(launch_general_cijk_lane(G, s));
+      G = sub(kMGenB);              // lp 5..6 a task per wave
       HIP_CHECK(launch_general_cijk(G, s));
     }
     if (mode == kIntegrate) {

@@ -624,6 +624,43 @@ __global__ __launch_bounds__(256) void general_cijk_kernel(const Task *__restric
   }
 }
 
+// The same change of variables with a task per lane, for lp <= 4 (the
+// matrix rows are the same for every lane of a level and lp: broadcast
+// loads); lanes of larger lp return (general_cijk_kernel takes them).
+template <int LP>
+__device__ __forceinline__ void lane_cijk(const PointParams &P, const Task &T, double *io) {
+  constexpr int NC = (LP + 1) * (LP + 2) * (LP + 3) / 6;
+  double cx[NC];
+#pragma unroll
+  for (int i = 0; i < NC; i++) cx[i] = io[i];
+  const double *MT = P.gmat + P.gmat_off[T.level * kGmatSlots + LP] + NC * NC;
+#pragma unroll
+  for (int o = 0; o < NC; o++) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NC; i++) s = fma(MT[i * NC + o], cx[i], s);
+    io[o] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void general_cijk_lane_kernel(const Task *__restrict__ tasks,
+                                                                PointParams P) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.ntasks) return;
+  const long long task = launch_task(P, t);
+  const Task &T = tasks[task];
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  double *io = P.cxyz_all + task * P.cxyz_stride;
+  switch (lp) {
+    case 0: lane_cijk<0>(P, T, io); break;
+    case 1: lane_cijk<1>(P, T, io); break;
+    case 2: lane_cijk<2>(P, T, io); break;
+    case 3: lane_cijk<3>(P, T, io); break;
+    case 4: lane_cijk<4>(P, T, io); break;
+    default: break;
+  }
+}
+
 // Collocation of general tasks, one task per wave, into the HBM level grids.
 template <int LPB, typename R>
 __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__restrict__ tasks,
@@ -764,6 +801,14 @@ static hipError_t launch_gen(K kernel, const PointParams &P, hipStream_t s) {
 hipError_t launch_general_cijk(const PointParams &P, hipStream_t s) {
   if (P.ntasks <= 0) return hipSuccess;
   hipLaunchKernelGGL(general_cijk_kernel, dim3((P.ntasks + 3) / 4), dim3(256), 0, s, P.tasks, P);
+  return hipGetLastError();
+}
+
+// General tasks with lp <= 4 (the others in the range are left alone).
+hipError_t launch_general_cijk_lane(const PointParams &P, hipStream_t s) {
+  if (P.ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(general_cijk_lane_kernel, dim3((P.ntasks + 255) / 256), dim3(256), 0, s,
+                     P.tasks, P);
   return hipGetLastError();
 }
 
