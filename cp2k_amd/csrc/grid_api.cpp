@@ -38,10 +38,13 @@ hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab
                              long long cart_len, int cach, int max_nsb, hipStream_t s);
 hipError_t launch_decontract_staged(const Group *groups, int ngroups, const double *pab,
                                     const double *sphi, double *cart, long long cart_len, int nc,
-                                    int wave_doubles, int nwaves, int blocks, hipStream_t s);
+                                    int wave_doubles, int nwaves, int blocks, bool pf,
+                                    hipStream_t s);
 hipError_t launch_contract_staged(const Group *groups, int ngroups, const double *hab_cart,
                                   const double *sphi, double *hab, long long hab_len, int nc,
-                                  int wave_doubles, int nwaves, int blocks, hipStream_t s);
+                                  int wave_doubles, int nwaves, int blocks, bool pf,
+                                  hipStream_t s);
+int stage_regs();
 hipError_t launch_contract(const Group *groups, int ngroups, const double *hab_cart,
                            const double *sphi, double *hab, long long hab_len, int cach,
                            int max_nsb, int max_nsa_nsb, hipStream_t s);
@@ -2131,6 +2134,7 @@ int decontract_chunk(const TaskListImpl *tl) {
 // does not fit (the unstaged kernels take those lists).
 struct StagePlan {
   bool ok = false;
+  bool pf_d = false, pf_c = false;  // operands fit the prefetch registers
   int nc_d = 1, wd_d = 0, nc_c = 1, wd_c = 0, nwaves = 4, blocks_d = 1, blocks_c = 1;
 };
 
@@ -2156,6 +2160,10 @@ StagePlan stage_plan(const TaskListImpl *tl) {
   };
   p.blocks_d = num_cus() * resident(p.wd_d);
   p.blocks_c = num_cus() * resident(p.wd_c);
+  const long long cap = 64LL * stage_regs();
+  p.pf_d = A * B <= cap && B * CB <= cap && A * std::min<long long>(p.nc_d, CA) <= cap;
+  p.pf_c = CB * std::min<long long>(p.nc_c, CA) <= cap && B * CB <= cap &&
+           A * std::min<long long>(p.nc_c, CA) <= cap;
   p.ok = true;
   return p;
 }
@@ -2168,7 +2176,7 @@ void run_decontract(TaskListImpl *tl, const double *pab, hipStream_t s,
     HIP_CHECK(launch_decontract_staged((const Group *)tl->d_groups.p, tl->ngroups, pab,
                                        (const double *)tl->d_sphi.p,
                                        cart ? cart : (double *)tl->d_cart.p, tl->cart_len,
-                                       sp.nc_d, sp.wd_d, sp.nwaves, sp.blocks_d, s));
+                                       sp.nc_d, sp.wd_d, sp.nwaves, sp.blocks_d, sp.pf_d, s));
     return;
   }
   const int cach = decontract_chunk(tl);
@@ -2280,7 +2288,7 @@ void run_integrate(TaskListImpl *tl, bool tau, const double *pab, const double *
       HIP_CHECK(launch_contract_staged((const Group *)tl->d_groups.p, tl->ngroups,
                                        (const double *)tl->d_hab_cart.p,
                                        (const double *)tl->d_sphi.p, hab, hab_len, sp.nc_c,
-                                       sp.wd_c, sp.nwaves, sp.blocks_c, s));
+                                       sp.wd_c, sp.nwaves, sp.blocks_c, sp.pf_c, s));
     } else {
       const int cach = decontract_chunk(tl);
       HIP_CHECK(launch_contract((const Group *)tl->d_groups.p, tl->ngroups,

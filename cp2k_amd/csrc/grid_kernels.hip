@@ -848,6 +848,76 @@ __global__ __launch_bounds__(64 * kGroupWaves) void contract_kernel(
 //               acc[sb][sa] += sum_c W[sb][c] Sa[sa][c], hab = acc
 // LDS per wave (doubles): decontract nsa nsb + nsb ncob + nc (nsa + nsb);
 // contract nsb ncob + nc (ncob + nsb + nsa) + nsa nsb.
+// Operand staging of one group: element e of each matrix (lane + 64 u) is
+// loaded into registers (all loads in flight together, while the previous
+// group computes) and later written to the wave's LDS.  kStageRegs elements
+// per lane and matrix; the host uses the kernels only when every matrix of
+// the list fits.
+constexpr int kStageRegs = 8;
+
+struct StageRegs {
+  double p[kStageRegs], b[kStageRegs], a[kStageRegs];
+};
+
+// decontract: P[sa][sb] from the block, Sb[sb][cb], Sa[sa][c] (first chunk)
+__device__ __forceinline__ void dc_load(const Group &g, const double *pab, const double *sphi,
+                                        int nc, int lane, StageRegs &R) {
+  const double *blk = pab + g.block_off;
+#pragma unroll
+  for (int u = 0; u < kStageRegs; u++) {
+    const int e = lane + 64 * u;
+    if (e < g.nsa * g.nsb) {
+      if (g.transpose) {
+        const int sb = e / g.nsa, sa = e - sb * g.nsa;
+        R.p[u] = blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa];
+      } else {
+        const int sa = e / g.nsb, sb = e - sa * g.nsb;
+        R.p[u] = blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
+      }
+    }
+    if (e < g.nsb * g.ncob) {
+      const int sb = e / g.ncob, cb = e - sb * g.ncob;
+      R.b[u] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+    }
+    if (e < g.nsa * nc) {
+      const int sa = e / nc, c = e - sa * nc;
+      R.a[u] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c];
+    }
+  }
+}
+
+__device__ __forceinline__ void dc_store(const Group &g, int nc, int lane, const StageRegs &R,
+                                         double *P, double *Sb, double *Sa) {
+#pragma unroll
+  for (int u = 0; u < kStageRegs; u++) {
+    const int e = lane + 64 * u;
+    if (e < g.nsa * g.nsb) {
+      if (g.transpose) {
+        const int sb = e / g.nsa, sa = e - sb * g.nsa;
+        P[sa * g.nsb + sb] = R.p[u];
+      } else {
+        P[e] = R.p[u];
+      }
+    }
+    if (e < g.nsb * g.ncob) Sb[e] = R.b[u];
+    if (e < g.nsa * nc) Sa[e] = R.a[u];
+  }
+}
+
+// Staged (de)contraction: persistent waves, each taking groups
+// g = wave, wave + total waves, ...; every operand is copied into the wave's
+// LDS (the block's (sa, sb) sub-block P, the rows sb of sphi_b, chunks of nc
+// columns of the rows sa of sphi_a; contract: the hab_cart chunk H), then
+// both products run from LDS:
+//   decontract  W[sb][c] = sum_sa P[sa][sb] Sa[sa][c],
+//               cart[cb][c] = sum_sb Sb[sb][cb] W[sb][c]
+//   contract    W[sb][c] = sum_cb Sb[sb][cb] H[cb][c]
+//               acc[sb][sa] += sum_c W[sb][c] Sa[sa][c], hab = acc
+// PF: the operands of the first chunk of the next group are loaded into
+// registers before the current group computes (software pipeline).
+// LDS per wave (doubles): decontract nsa nsb + nsb ncob + nc (nsa + nsb);
+// contract nsb ncob + nc (ncob + nsb + nsa) + nsa nsb.
+template <bool PF>
 __global__ __launch_bounds__(256) void decontract_staged_kernel(
     const Group *__restrict__ groups, int ngroups, const double *__restrict__ pab,
     const double *__restrict__ sphi, double *__restrict__ cart, long long cart_len, int nc_max,
@@ -856,35 +926,50 @@ __global__ __launch_bounds__(256) void decontract_staged_kernel(
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
+  const int stride = gridDim.x * nw;
   double *base = lds_ds + (size_t)wave * wave_doubles;
-  for (int gi = blockIdx.x * nw + wave; gi < ngroups; gi += gridDim.x * nw) {
-    const Group g = groups[gi];
+  int gi = blockIdx.x * nw + wave;
+  if (gi >= ngroups) return;
+  Group g = groups[gi];
+  StageRegs R;
+  if constexpr (PF) dc_load(g, pab, sphi, min(nc_max, g.ncoa), lane, R);
+  while (true) {
     const int nsa = g.nsa, nsb = g.nsb, ncoa = g.ncoa, ncob = g.ncob;
     double *P = base;                   // [nsa][nsb]
     double *Sb = P + nsa * nsb;         // [nsb][ncob]
     double *Sa = Sb + nsb * ncob;       // [nsa][nc]
     double *W = Sa + nsa * nc_max;      // [nsb][nc]
-    const double *blk = pab + g.block_off;
-    if (g.transpose) {
-      for (int e = lane; e < nsa * nsb; e += 64) {
-        const int sb = e / nsa, sa = e - sb * nsa;
-        P[sa * nsb + sb] = blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa];
-      }
+    if constexpr (PF) {
+      dc_store(g, min(nc_max, ncoa), lane, R, P, Sb, Sa);
     } else {
+      const double *blk = pab + g.block_off;
       for (int e = lane; e < nsa * nsb; e += 64) {
-        const int sa = e / nsb, sb = e - sa * nsb;
-        P[e] = blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
+        if (g.transpose) {
+          const int sb = e / nsa, sa = e - sb * nsa;
+          P[sa * nsb + sb] = blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa];
+        } else {
+          const int sa = e / nsb, sb = e - sa * nsb;
+          P[e] = blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
+        }
+      }
+      for (int e = lane; e < nsb * ncob; e += 64) {
+        const int sb = e / ncob, cb = e - sb * ncob;
+        Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
       }
     }
-    for (int e = lane; e < nsb * ncob; e += 64) {
-      const int sb = e / ncob, cb = e - sb * ncob;
-      Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+    const int gn = gi + stride;
+    Group g2;
+    if (gn < ngroups) {
+      g2 = groups[gn];
+      if constexpr (PF) dc_load(g2, pab, sphi, min(nc_max, g2.ncoa), lane, R);
     }
     for (int c0 = 0; c0 < ncoa; c0 += nc_max) {
       const int nc = min(nc_max, ncoa - c0);
-      for (int e = lane; e < nsa * nc; e += 64) {
-        const int sa = e / nc, c = e - sa * nc;
-        Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
+      if (!PF || c0 > 0) {
+        for (int e = lane; e < nsa * nc; e += 64) {
+          const int sa = e / nc, c = e - sa * nc;
+          Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
+        }
       }
       wave_sync();
       for (int e = lane; e < nsb * nc; e += 64) {
@@ -903,9 +988,34 @@ __global__ __launch_bounds__(256) void decontract_staged_kernel(
       }
       wave_sync();
     }
+    if (gn >= ngroups) break;
+    g = g2;
+    gi = gn;
   }
 }
 
+// contract: H[cb][c] (first chunk), Sb[sb][cb], Sa[sa][c] (first chunk)
+__device__ __forceinline__ void ct_load(const Group &g, const double *hab_cart,
+                                        const double *sphi, int nc, int lane, StageRegs &R) {
+#pragma unroll
+  for (int u = 0; u < kStageRegs; u++) {
+    const int e = lane + 64 * u;
+    if (e < g.ncob * nc) {
+      const int cb = e / nc, c = e - cb * nc;
+      R.p[u] = hab_cart[g.cart_off + (long long)cb * g.ncoa + c];
+    }
+    if (e < g.nsb * g.ncob) {
+      const int sb = e / g.ncob, cb = e - sb * g.ncob;
+      R.b[u] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+    }
+    if (e < g.nsa * nc) {
+      const int sa = e / nc, c = e - sa * nc;
+      R.a[u] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c];
+    }
+  }
+}
+
+template <bool PF>
 __global__ __launch_bounds__(256) void contract_staged_kernel(
     const Group *__restrict__ groups, int ngroups, const double *__restrict__ hab_cart,
     const double *__restrict__ sphi, double *__restrict__ hab, long long hab_len, int nc_max,
@@ -914,9 +1024,14 @@ __global__ __launch_bounds__(256) void contract_staged_kernel(
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
+  const int stride = gridDim.x * nw;
   double *base = lds_cs + (size_t)wave * wave_doubles;
-  for (int gi = blockIdx.x * nw + wave; gi < ngroups; gi += gridDim.x * nw) {
-    const Group g = groups[gi];
+  int gi = blockIdx.x * nw + wave;
+  if (gi >= ngroups) return;
+  Group g = groups[gi];
+  StageRegs R;
+  if constexpr (PF) ct_load(g, hab_cart, sphi, min(nc_max, g.ncoa), lane, R);
+  while (true) {
     const int nsa = g.nsa, nsb = g.nsb, ncoa = g.ncoa, ncob = g.ncob;
     double *acc = base;                 // [nsb][nsa]
     double *Sb = acc + nsa * nsb;       // [nsb][ncob]
@@ -924,19 +1039,38 @@ __global__ __launch_bounds__(256) void contract_staged_kernel(
     double *W = H + ncob * nc_max;      // [nsb][nc]
     double *Sa = W + nsb * nc_max;      // [nsa][nc]
     for (int e = lane; e < nsa * nsb; e += 64) acc[e] = 0.0;
-    for (int e = lane; e < nsb * ncob; e += 64) {
-      const int sb = e / ncob, cb = e - sb * ncob;
-      Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+    if constexpr (PF) {
+      const int nc = min(nc_max, ncoa);
+#pragma unroll
+      for (int u = 0; u < kStageRegs; u++) {
+        const int e = lane + 64 * u;
+        if (e < ncob * nc) H[e] = R.p[u];
+        if (e < nsb * ncob) Sb[e] = R.b[u];
+        if (e < nsa * nc) Sa[e] = R.a[u];
+      }
+    } else {
+      for (int e = lane; e < nsb * ncob; e += 64) {
+        const int sb = e / ncob, cb = e - sb * ncob;
+        Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
+      }
+    }
+    const int gn = gi + stride;
+    Group g2;
+    if (gn < ngroups) {
+      g2 = groups[gn];
+      if constexpr (PF) ct_load(g2, hab_cart, sphi, min(nc_max, g2.ncoa), lane, R);
     }
     for (int c0 = 0; c0 < ncoa; c0 += nc_max) {
       const int nc = min(nc_max, ncoa - c0);
-      for (int e = lane; e < ncob * nc; e += 64) {
-        const int cb = e / nc, c = e - cb * nc;
-        H[e] = hab_cart[g.cart_off + (long long)cb * ncoa + c0 + c];
-      }
-      for (int e = lane; e < nsa * nc; e += 64) {
-        const int sa = e / nc, c = e - sa * nc;
-        Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
+      if (!PF || c0 > 0) {
+        for (int e = lane; e < ncob * nc; e += 64) {
+          const int cb = e / nc, c = e - cb * nc;
+          H[e] = hab_cart[g.cart_off + (long long)cb * ncoa + c0 + c];
+        }
+        for (int e = lane; e < nsa * nc; e += 64) {
+          const int sa = e / nc, c = e - sa * nc;
+          Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
+        }
       }
       wave_sync();
       for (int e = lane; e < nsb * nc; e += 64) {
@@ -961,6 +1095,9 @@ __global__ __launch_bounds__(256) void contract_staged_kernel(
       if (q < hab_len) hab[q] = acc[e];
     }
     wave_sync();
+    if (gn >= ngroups) break;
+    g = g2;
+    gi = gn;
   }
 }
 
@@ -1090,31 +1227,43 @@ hipError_t launch_integrate(const PointParams &P, bool ortho, hipStream_t s) {
 
 // Staged kernels: nwaves waves per workgroup of wave_doubles each, at most
 // `blocks` workgroups (persistent: the waves loop over the groups).
+// pf: the list's matrices fit kStageRegs elements per lane (prefetching
+// kernels).
 hipError_t launch_decontract_staged(const Group *groups, int ngroups, const double *pab,
                                     const double *sphi, double *cart, long long cart_len, int nc,
-                                    int wave_doubles, int nwaves, int blocks, hipStream_t s) {
+                                    int wave_doubles, int nwaves, int blocks, bool pf,
+                                    hipStream_t s) {
   if (ngroups == 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)wave_doubles * nwaves;
-  const hipError_t e = allow_lds(decontract_staged_kernel, lds);
-  if (e != hipSuccess) return e;
   blocks = std::max(1, std::min(blocks, (ngroups + nwaves - 1) / nwaves));
-  hipLaunchKernelGGL(decontract_staged_kernel, dim3(blocks), dim3(64 * nwaves), lds, s, groups,
-                     ngroups, pab, sphi, cart, cart_len, nc, wave_doubles);
-  return hipGetLastError();
+  auto go = [&](auto kernel) {
+    const hipError_t e = allow_lds(kernel, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * nwaves), lds, s, groups, ngroups, pab,
+                       sphi, cart, cart_len, nc, wave_doubles);
+    return hipGetLastError();
+  };
+  return pf ? go(decontract_staged_kernel<true>) : go(decontract_staged_kernel<false>);
 }
 
 hipError_t launch_contract_staged(const Group *groups, int ngroups, const double *hab_cart,
                                   const double *sphi, double *hab, long long hab_len, int nc,
-                                  int wave_doubles, int nwaves, int blocks, hipStream_t s) {
+                                  int wave_doubles, int nwaves, int blocks, bool pf,
+                                  hipStream_t s) {
   if (ngroups == 0) return hipSuccess;
   const size_t lds = sizeof(double) * (size_t)wave_doubles * nwaves;
-  const hipError_t e = allow_lds(contract_staged_kernel, lds);
-  if (e != hipSuccess) return e;
   blocks = std::max(1, std::min(blocks, (ngroups + nwaves - 1) / nwaves));
-  hipLaunchKernelGGL(contract_staged_kernel, dim3(blocks), dim3(64 * nwaves), lds, s, groups,
-                     ngroups, hab_cart, sphi, hab, hab_len, nc, wave_doubles);
-  return hipGetLastError();
+  auto go = [&](auto kernel) {
+    const hipError_t e = allow_lds(kernel, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64 * nwaves), lds, s, groups, ngroups,
+                       hab_cart, sphi, hab, hab_len, nc, wave_doubles);
+    return hipGetLastError();
+  };
+  return pf ? go(contract_staged_kernel<true>) : go(contract_staged_kernel<false>);
 }
+
+int stage_regs() { return kStageRegs; }
 
 hipError_t launch_decontract(const Group *groups, int ngroups, const double *pab,
                              long long pab_len, const double *sphi, double *cart,
