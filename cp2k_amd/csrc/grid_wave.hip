@@ -583,14 +583,18 @@ __device__ void task_finish(const PointParams &P, const Task &T, const Group &G,
   if (FULL && P.do_forces) {
     double *out = P.task_out + task * kTaskOutStride;
     const double scale = (T.iatom == T.jatom) ? 1.0 : 2.0;
-    for (int c = 0; c < kTaskOutStride; c++) {
+    // without virial only the 6 force sums (the 9 virial slots are zero)
+    const int nsum = P.do_virial ? kTaskOutStride : 6;
+    if (NT == 64 && !P.do_virial && lane >= 6 && lane < kTaskOutStride) out[lane] = 0.0;
+    if (NT > 64 && !P.do_virial && tid >= 6 && tid < kTaskOutStride) out[tid] = 0.0;
+    for (int c = 0; c < nsum; c++) {
       const double s = wave_total(c < 6 ? f[c] : v[c - 6]);
       if (NT == 64 && lane == 63) out[c] = scale * s;
       if (NT > 64 && lane == 63) red[(tid >> 6) * kTaskOutStride + c] = s;
     }
     if constexpr (NT > 64) {
       __syncthreads();
-      if (tid < kTaskOutStride) {
+      if (tid < nsum) {
         double s = 0.0;
         for (int w = 0; w < NT / 64; w++) s += red[w * kTaskOutStride + tid];
         out[tid] = scale * s;
