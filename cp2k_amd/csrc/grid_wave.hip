@@ -1087,25 +1087,16 @@ __global__ __launch_bounds__(256) void collocate_wave_kernel(const Task *__restr
 // the items U.ibeg + U.first + m * U.stride < U.iend of one tile.  The tile
 // accumulates in FP64 whatever the evaluation type.
 template <int LPB, typename R>
-__global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
-    const Task *__restrict__ tasks, const Level *__restrict__ levels,
-    const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
+__device__ __forceinline__ void collocate_tile_unit(const Task *__restrict__ tasks,
+                                                    const Level *__restrict__ levels,
+                                                    const TileUnit *__restrict__ units,
+                                                    const TileItem *__restrict__ items,
+                                                    const PointParams &P, int u) {
   extern __shared__ double lds[];
   const WavePlan &wp = P.wp;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
-  // one unit per workgroup, or (work queue) units from the cursor until
-  // all are taken
-  __shared__ int next_unit;
-  int u = blockIdx.x;
-  if (P.queue) {
-    if (threadIdx.x == 0) next_unit = atomicAdd(P.queue, 1);
-    __syncthreads();
-    u = next_unit;
-    __syncthreads();
-  }
-  while (u < P.nunits) {
   const TileUnit U = units[P.unit_base + u];
   const Level &L = levels[U.level];
   const Layout &lay = L.lay;
@@ -1150,66 +1141,93 @@ __global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
       if (v != 0.0) atomicAdd(&dst[x], v);
     }
   }
-  if (!P.queue) break;
-  __syncthreads();  // the window is cleared for the next unit
-  if (threadIdx.x == 0) next_unit = atomicAdd(P.queue, 1);
-  __syncthreads();
-  u = next_unit;
-  __syncthreads();
+}
+
+// One unit per workgroup; QUEUE (persistent launch of a batch): units from
+// the cursor until all are taken.
+template <int LPB, typename R, bool QUEUE>
+__global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void collocate_tile_kernel(
+    const Task *__restrict__ tasks, const Level *__restrict__ levels,
+    const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
+  if constexpr (!QUEUE) {
+    collocate_tile_unit<LPB, R>(tasks, levels, units, items, P, blockIdx.x);
+  } else {
+    __shared__ int next_unit;
+    if (threadIdx.x == 0) next_unit = atomicAdd(P.queue, 1);
+    __syncthreads();
+    int u = next_unit;
+    __syncthreads();
+    while (u < P.nunits) {
+      collocate_tile_unit<LPB, R>(tasks, levels, units, items, P, u);
+      __syncthreads();  // the window is cleared for the next unit
+      if (threadIdx.x == 0) next_unit = atomicAdd(P.queue, 1);
+      __syncthreads();
+      u = next_unit;
+      __syncthreads();
+    }
   }
 }
 
 // R = float reads the FP32 copy of the potential (Level::gridf).
 template <int LPB, typename R>
+__device__ __forceinline__ void integrate_wave_one(const Task *__restrict__ tasks,
+                                                   const Level *__restrict__ levels,
+                                                   const PointParams &P, R *tab, long long t,
+                                                   int lane) {
+  const WavePlan &wp = P.wp;
+  const long long task = launch_task(P, t);
+  const Task &T = tasks[task];
+  const GeoClass &G = P.geo[T.geo];
+  const Level &L = levels[T.level];
+  const Layout &lay = L.lay;
+  const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
+  wave_tables(T, lay, lp, wp, tab, lane);
+  double *out = P.cxyz_all + task * P.cxyz_stride;
+  const R *grid;
+  if constexpr (sizeof(R) == 4)
+    grid = L.gridf;
+  else
+    grid = L.grid;
+  switch (lp) {
+    case 0: integrate_task<0>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 1: integrate_task<1>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 2: integrate_task<2>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 3: integrate_task<3>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 4: integrate_task<4>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
+    default: break;  // the host routes larger lp to the per-task kernels
+  }
+}
+
+// One task per wave; QUEUE (persistent launch of a batch): groups of qchunk
+// tasks from the cursor until all are taken.  Two instantiations so that the
+// one-task launch keeps the straight-line code of a single task.
+template <int LPB, typename R, bool QUEUE>
 __global__ __launch_bounds__(256, (LPB <= 4 ? 3 : 2)) void integrate_wave_kernel(
     const Task *__restrict__ tasks, const Level *__restrict__ levels, PointParams P) {
   extern __shared__ double lds[];
-  const WavePlan &wp = P.wp;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // one task per wave, or (work queue) groups of qchunk tasks from the
-  // cursor until all are taken
-  long long t0 = (long long)blockIdx.x * (blockDim.x >> 6) + wave, t1 = t0 + 1;
-  if (P.queue) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(P.queue, P.qchunk);
-    t0 = __builtin_amdgcn_readfirstlane(base);
-    t1 = t0 + P.qchunk;
-  }
-  const WaveLds W = wave_lds(lds, wp, wave);
+  const WaveLds W = wave_lds(lds, P.wp, wave);
   R *tab = reinterpret_cast<R *>(W.u);
-  while (t0 < P.ntasks) {
-    for (long long t = t0; t < t1 && t < P.ntasks; t++) {
-      const long long task = launch_task(P, t);
-      const Task &T = tasks[task];
-      const GeoClass &G = P.geo[T.geo];
-      const Level &L = levels[T.level];
-      const Layout &lay = L.lay;
-      const int lp = T.la_max + T.lb_max + P.ld[0] + P.ld[2];
-      wave_tables(T, lay, lp, wp, tab, lane);
-      double *out = P.cxyz_all + task * P.cxyz_stride;
-      const R *grid;
-      if constexpr (sizeof(R) == 4)
-        grid = L.gridf;
-      else
-        grid = L.grid;
-      switch (lp) {
-        case 0: integrate_task<0>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        case 1: integrate_task<1>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        case 2: integrate_task<2>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        case 3: integrate_task<3>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        case 4: integrate_task<4>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        case 5: if (LPB >= 5) integrate_task<(LPB >= 5 ? 5 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        case 6: if (LPB >= 6) integrate_task<(LPB >= 6 ? 6 : 0)>(P, T, G, lay, wp, tab, grid, out, lane); break;
-        default: break;  // the host routes larger lp to the per-task kernels
-      }
-      wave_sync();  // the tables are rewritten for the next task
-    }
-    if (!P.queue) break;
+  if constexpr (!QUEUE) {
+    const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (t >= P.ntasks) return;
+    integrate_wave_one<LPB, R>(tasks, levels, P, tab, t, lane);
+  } else {
     int base = 0;
     if (lane == 0) base = atomicAdd(P.queue, P.qchunk);
-    t0 = __builtin_amdgcn_readfirstlane(base);
-    t1 = t0 + P.qchunk;
+    long long t0 = __builtin_amdgcn_readfirstlane(base);
+    while (t0 < P.ntasks) {
+      for (long long t = t0; t < t0 + P.qchunk && t < P.ntasks; t++) {
+        integrate_wave_one<LPB, R>(tasks, levels, P, tab, t, lane);
+        wave_sync();  // the tables are rewritten for the next task
+      }
+      base = 0;
+      if (lane == 0) base = atomicAdd(P.queue, P.qchunk);
+      t0 = __builtin_amdgcn_readfirstlane(base);
+    }
   }
 }
 
@@ -1358,9 +1376,15 @@ static hipError_t collocate_tiles_r(const PointParams &P, int lpb, int nunits, h
                                : nunits;
     return launch_wave(kernel, P, blocks, s, P.tasks, P.levels, P.units, P.items);
   };
-  if (lpb == 4) return go(collocate_tile_kernel<4, R>);
-  if (lpb == 6) return go(collocate_tile_kernel<6, R>);
-  if (lpb == 8) return go(collocate_tile_kernel<8, R>);
+  if (P.queue) {
+    if (lpb == 4) return go(collocate_tile_kernel<4, R, true>);
+    if (lpb == 6) return go(collocate_tile_kernel<6, R, true>);
+    if (lpb == 8) return go(collocate_tile_kernel<8, R, true>);
+  } else {
+    if (lpb == 4) return go(collocate_tile_kernel<4, R, false>);
+    if (lpb == 6) return go(collocate_tile_kernel<6, R, false>);
+    if (lpb == 8) return go(collocate_tile_kernel<8, R, false>);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -1384,8 +1408,13 @@ static hipError_t integrate_wave_r(const PointParams &P, int lpb, hipStream_t s)
                                : wave_blocks(P);
     return launch_wave(kernel, P, blocks, s, P.tasks, P.levels);
   };
-  if (lpb == 4) return go(integrate_wave_kernel<4, R>);
-  if (lpb == 6) return go(integrate_wave_kernel<6, R>);
+  if (P.queue) {
+    if (lpb == 4) return go(integrate_wave_kernel<4, R, true>);
+    if (lpb == 6) return go(integrate_wave_kernel<6, R, true>);
+  } else {
+    if (lpb == 4) return go(integrate_wave_kernel<4, R, false>);
+    if (lpb == 6) return go(integrate_wave_kernel<6, R, false>);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -1423,12 +1452,15 @@ hipError_t launch_finish_block(const PointParams &P, hipStream_t s) {
   return full ? go(finish_block_kernel<1>) : go(finish_block_kernel<0>);
 }
 
-// The lane kernel serves calls without virial and tau for la_max + lb_max
-// <= 3 (lp: after the call's shift); false: use the wave kernel.
+// The lane kernel serves calls with forces, without virial and tau, for
+// la_max + lb_max <= 3 (lp: after the call's shift); false: use the wave
+// kernel.
 bool finish_lane_ok(const PointParams &P, int lp) {
   const int lp0 = lp - P.ld[0] - P.ld[2];
-  return !P.do_virial && !P.compute_tau && lp0 <= 3 && P.ld[2] == 0 &&
-         P.ld[0] == (P.do_forces ? 1 : 0);
+  // without forces the wave kernel measured faster (hab only: few loads per
+  // task, the lane kernel's scattered atomics dominate)
+  return P.do_forces && !P.do_virial && !P.compute_tau && lp0 <= 3 && P.ld[2] == 0 &&
+         P.ld[0] == 1;
 }
 
 hipError_t launch_finish_lane(const PointParams &P, hipStream_t s) {
