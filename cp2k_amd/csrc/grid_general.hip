@@ -226,9 +226,14 @@ __device__ __forceinline__ int gen_tile_stride(int nx) { return nx | 1; }
 // Collocate the rows j0..j1, planes k0..k1 of a task (an item) into the LDS
 // tile of unit U: win[((gz - gz0) * by + gy - gy0) * sx + gx].
 template <int LP, typename R>
-__device__ __forceinline__ void gen_tile_rows(const Task &T, const Layout &lay, const double *cijk,
+__device__ __forceinline__ void gen_tile_rows(const Task &Tg, const Layout &layg, const double *cijk,
                                               int NP, const TileItem &I, const TileUnit &U,
                                               double *win, int lane) {
+  // the task and layout in registers: the tile's atomics could otherwise
+  // alias them, and every row would reload them and recompute what does not
+  // depend on the row
+  const Task T = Tg;
+  const Layout lay = layg;
   const int nj = I.j1 - I.j0 + 1, nk = I.k1 - I.k0 + 1;
   const int sx = gen_tile_stride(lay.npts_local[0]);
   double hh = 0.0;
