@@ -683,6 +683,10 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     }
     if ((long long)y.npts_local[0] * y.npts_local[1] * y.npts_local[2] >= (1LL << 31))
       fatal("level %d: local grids of 2^31 points or more are not supported", l + 1);
+    // the general-cell integrate addresses a point inside its z plane with a
+    // 32-bit byte offset
+    if ((long long)y.npts_local[0] * y.npts_local[1] >= (1LL << 29))
+      fatal("level %d: local grid planes of 2^29 points or more are not supported", l + 1);
   }
 
   // --- validate indices (the reference trusts its caller; a wrong index here

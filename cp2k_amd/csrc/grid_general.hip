@@ -414,6 +414,13 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
   const double qq = exp(-2.0 * T.zetp * g22);
   const int nx = lay.npts_local[0], nxy = nx * lay.npts_local[1];
   const int ngx = lay.npts_global[0], ngy = lay.npts_global[1], ngz = lay.npts_global[2];
+  // the level grid as a global-memory pointer: generic (flat) loads would
+  // also count on the LDS counter, and every wait for a row-table read would
+  // then drain the potential loads in flight
+  typedef const __attribute__((address_space(1))) char GByte;
+  typedef const __attribute__((address_space(1))) R GVal;
+  GByte *gg = (GByte *)grid;
+  const size_t plane_bytes = (size_t)nxy * sizeof(R);
   for (int kb = 0; kb < nk; kb += ks) {
     const int kn = min(ks, nk - kb);
     wave_sync();  // the previous slab's readers are done
@@ -468,15 +475,17 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
       const int gx = wrap(i - lay.shift_local[0], ngx);
       const int gy = wrap(j - lay.shift_local[1], ngy);
       const bool act = c < ncol && gx >= T.blo[0] && gx <= T.bhi[0];
-      const int klo = wave_min(act ? J.z : kn);
-      const int khi = -wave_min(act ? -J.w : 1);
+      // wave-uniform plane range (scalar registers: the plane loop and the
+      // plane base addresses stay off the vector ALU)
+      const int klo = __builtin_amdgcn_readfirstlane(wave_min(act ? J.z : kn));
+      const int khi = __builtin_amdgcn_readfirstlane(-wave_min(act ? -J.w : 1));
       const double di = i - T.gp[0], dj = j - T.gp[1];
       R M[LP + 1];
 #pragma unroll
       for (int l = 0; l <= LP; l++) M[l] = R(0);
       if (klo <= khi) {
         const double qij = W.q[x] * W.q[ni + jj];
-        const int cb = gy * nx + gx;
+        const unsigned cb = (unsigned)(gy * nx + gx) * (unsigned)sizeof(R);  // bytes into a plane
         // the Gaussian recurrence stays FP64 in both modes (in FP32 its
         // relative drift grows with the square of the planes walked)
         for (int k0 = klo; k0 <= khi; k0 += kColRestart) {
@@ -491,7 +500,7 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
           }
           double g = exp(-T.zetp * r2);
           double q = qij * W.q[ni + nj + kb + k0];
-          double dk = dk0;
+          double dk = dk0;  // FP64, rounded to R per point
           int gz = wrap(kabs - lay.shift_local[2], ngz);
           const int *rowk = W.rows + k0 * nj + jj;
           auto step = [&](R val) {
@@ -506,30 +515,36 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
             q *= qq;
             dk += 1.0;
           };
+          // row-table entry v -> is the column's point in the row; the table
+          // read is unconditional (every lane's rowk lies inside the table),
+          // so that no branch and no wait separates the loads of a batch
+          auto inside = [&](int v) {
+            return act & ((unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16));
+          };
+          // NB planes at a time, their loads issued together with no branch
+          // between them (a point outside the row reads the plane's first
+          // value and drops it): batches of kColBatch, then at most one
+          // batch each of 4, 2 and 1 for the planes left
           int kk = k0;
-          // kColBatch planes at a time: their loads are unconditional (index
-          // 0 stands in for a point outside the row) and in flight together
-          for (; kk + kColBatch - 1 <= k1; kk += kColBatch) {
-            R val[kColBatch];
+          auto batch = [&](auto nb) {
+            constexpr int NB = decltype(nb)::value;
+            R val[NB];
+            bool in[NB];
 #pragma unroll
-            for (int u = 0; u < kColBatch; u++) {
-              const int v = rowk[u * nj];
-              const bool in = act && (unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16);
-              const R w = grid[in ? gz * nxy + cb : 0];
-              val[u] = in ? w : R(0);
+            for (int u = 0; u < NB; u++) {
+              in[u] = inside(rowk[u * nj]);
+              val[u] = *(GVal *)(gg + gz * plane_bytes + (in[u] ? cb : 0u));
               gz = (gz + 1 == ngz) ? 0 : gz + 1;
             }
-            rowk += kColBatch * nj;
+            rowk += NB * nj;
+            kk += NB;
 #pragma unroll
-            for (int u = 0; u < kColBatch; u++) step(val[u]);
-          }
-          for (; kk <= k1; kk++, rowk += nj) {
-            const int v = *rowk;
-            const bool in = act && (unsigned)(x - (v & 0xffff)) < (unsigned)(v >> 16);
-            const R w = grid[in ? gz * nxy + cb : 0];
-            step(in ? w : R(0));
-            gz = (gz + 1 == ngz) ? 0 : gz + 1;
-          }
+            for (int u = 0; u < NB; u++) step(in[u] ? val[u] : R(0));
+          };
+          while (kk + kColBatch - 1 <= k1) batch(std::integral_constant<int, kColBatch>());
+          if (kk + 3 <= k1) batch(std::integral_constant<int, 4>());
+          if (kk + 1 <= k1) batch(std::integral_constant<int, 2>());
+          if (kk <= k1) batch(std::integral_constant<int, 1>());
         }
       }
       fold(M, act ? di : 0.0, act ? dj : 0.0);

@@ -396,11 +396,15 @@ __device__ __forceinline__ void integrate_group(const PointParams &P, const Task
   }
   int gz = wrap(cw[2] + kmin, ngz);
   const int nk = 2 - 2 * kmin;
+  // the grid through a global-memory pointer: generic (flat) loads would also
+  // count on the LDS counter, so every wait for a table read would drain them
+  typedef const __attribute__((address_space(1))) R GVal;
+  GVal *gridg = (GVal *)grid;
   if (sizeof(R) == 4 && gz + nk <= nz && gz + nk <= ngz) {
     // FP32, every plane inside the local grid and no periodic wrap on the
     // way: no test or wrap per plane (FP64 measured slower with this copy)
     const R *pz = tabz + (kmin - T.lo[2]) * NP;
-    const R *gp = grid + gz * nxy;
+    GVal *gp = gridg + gz * nxy;
 #pragma unroll 4  // several grid loads in flight per wave
     for (int k = kmin; k <= 1 - kmin; k++, pz += NP, gp += nxy) {
       R g[NCL];
@@ -427,11 +431,11 @@ __device__ __forceinline__ void integrate_group(const PointParams &P, const Task
           // FP32: unconditional loads (index 0 stands in for a point outside
           // the column) keep the unrolled iterations' loads in flight; FP64
           // measured faster with the loads under the branch
-          const R v = grid[in ? gz * nxy + C[u].base : 0];
+          const R v = gridg[in ? gz * nxy + C[u].base : 0];
           g[u] = in ? v : R(0);
         } else {
           g[u] = 0;
-          if (in) g[u] = grid[gz * nxy + C[u].base];
+          if (in) g[u] = gridg[gz * nxy + C[u].base];
         }
       }
 #pragma unroll
@@ -498,7 +502,7 @@ __device__ __forceinline__ void task_sync() {
 // KIND: 0 hab only, 1 hab with forces / virial (no tau: the plain getters
 // only, which keeps the register count low), 2 anything (tau getters).
 template <int KIND, int NT>
-__device__ void task_finish(const PointParams &P, const Task &T, const Group &G, int la, int lb,
+__device__ __forceinline__ void task_finish(const PointParams &P, const Task &T, const Group &G, int la, int lb,
                             int la_min, int lb_min, const WaveLds &W, double *red,
                             long long task, int tid) {
   const int lane = tid & 63;
