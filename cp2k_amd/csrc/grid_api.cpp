@@ -683,10 +683,10 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
     }
     if ((long long)y.npts_local[0] * y.npts_local[1] * y.npts_local[2] >= (1LL << 31))
       fatal("level %d: local grids of 2^31 points or more are not supported", l + 1);
-    // the general-cell integrate addresses a point inside its z plane with a
-    // 32-bit byte offset
-    if ((long long)y.npts_local[0] * y.npts_local[1] >= (1LL << 29))
-      fatal("level %d: local grid planes of 2^29 points or more are not supported", l + 1);
+    // the general-cell integrate reads a z plane through a buffer resource
+    // whose range check needs planes below 2^31 bytes
+    if ((long long)y.npts_local[0] * y.npts_local[1] >= (1LL << 28))
+      fatal("level %d: local grid planes of 2^28 points or more are not supported", l + 1);
   }
 
   // --- validate indices (the reference trusts its caller; a wrong index here
@@ -1623,7 +1623,7 @@ WavePlan general_col_plan(const TaskListImpl *tl, int level, int lp) {
   w.o_u = nc;
   const int nd = nc + (lp > 6 ? 64 * 40 : 0);
   w.o_q = 8 * nd;
-  w.q_cap = ni + nj + nk;
+  w.q_cap = ni + nj + 2 * nk;  // qi, qj, qk, then dk per plane
   w.o_rows = w.o_q + 8 * w.q_cap;
   // a slab holds at least one plane; 1024 rows (4 KiB) keep the wave region
   // small (more waves per CU) and cover most tasks in one or two slabs

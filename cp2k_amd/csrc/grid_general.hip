@@ -321,6 +321,24 @@ __device__ __forceinline__ ColLds col_lds(double *lds, const WavePlan &wp, int w
   return w;
 }
 
+// Buffer-resource load of the value at byte offset off of a grid plane of
+// `bytes` bytes at `plane` (wave-uniform).  Offsets at or past `bytes` return
+// zero (the hardware's range check), kPlaneOutside among them: planes are
+// below 2^31 bytes.
+constexpr unsigned kPlaneOutside = 0x80000000u;
+constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
+
+template <typename R>
+__device__ __forceinline__ R plane_load(const char *plane, unsigned bytes, unsigned off) {
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(plane), (short)0, (int)bytes, kBufferWord3);
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+  }
+}
+
 // One column's moments -> the task's sums, lane-private (lp <= 6):
 // acc[(kl, jl, il)] += M[kl] dj^jl di^il.
 template <int LP, typename R>
@@ -394,7 +412,7 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
                                          const WavePlan &wp, const ColLds &W, int lane,
                                          Fold &&fold) {
   const int ni = T.hi[0] - T.lo[0] + 1, nj = T.hi[1] - T.lo[1] + 1, nk = T.hi[2] - T.lo[2] + 1;
-  if (nj > wp.j_cap || nj > wp.rows_cap || ni + nj + nk > wp.q_cap) return;  // host-sized: never
+  if (nj > wp.j_cap || nj > wp.rows_cap || ni + nj + 2 * nk > wp.q_cap) return;  // host-sized: never
   const int ks = min(nk, wp.rows_cap / nj);  // planes per slab
   const double *h0 = lay.dh[0], *h1 = lay.dh[1], *h2 = lay.dh[2];
   const double g02 = h0[0] * h2[0] + h0[1] * h2[1] + h0[2] * h2[2];
@@ -411,16 +429,20 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
       arg = 2.0 * g22 * (T.lo[2] + (e - ni - nj) - T.gp[2]) + g22;
     W.q[e] = exp(-T.zetp * arg);
   }
+  // dk of every plane in R (the same for all columns: a broadcast LDS read
+  // per point instead of an FP64 add and a conversion)
+  R *dkt = reinterpret_cast<R *>(W.q + ni + nj + nk);
+  for (int e = lane; e < nk; e += 64) dkt[e] = (R)(T.lo[2] + e - T.gp[2]);
   const double qq = exp(-2.0 * T.zetp * g22);
   const int nx = lay.npts_local[0], nxy = nx * lay.npts_local[1];
   const int ngx = lay.npts_global[0], ngy = lay.npts_global[1], ngz = lay.npts_global[2];
-  // the level grid as a global-memory pointer: generic (flat) loads would
-  // also count on the LDS counter, and every wait for a row-table read would
-  // then drain the potential loads in flight
-  typedef const __attribute__((address_space(1))) char GByte;
-  typedef const __attribute__((address_space(1))) R GVal;
-  GByte *gg = (GByte *)grid;
-  const size_t plane_bytes = (size_t)nxy * sizeof(R);
+  // potential loads through a buffer resource per z plane (base address in
+  // scalar registers, the point's byte offset in the plane per lane): a
+  // point outside the task reads at kPlaneOutside, past the plane, which the
+  // buffer range check returns as zero.  Buffer loads count on the vector
+  // memory counter only, so a wait for a row-table read does not drain them.
+  const char *gbase = reinterpret_cast<const char *>(grid);
+  const unsigned plane_bytes = (unsigned)nxy * (unsigned)sizeof(R);  // < 2^31 (host check)
   for (int kb = 0; kb < nk; kb += ks) {
     const int kn = min(ks, nk - kb);
     wave_sync();  // the previous slab's readers are done
@@ -500,12 +522,12 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
           }
           double g = exp(-T.zetp * r2);
           double q = qij * W.q[ni + nj + kb + k0];
-          double dk = dk0;  // FP64, rounded to R per point
           int gz = wrap(kabs - lay.shift_local[2], ngz);
+          const char *plane = gbase + (size_t)gz * plane_bytes;
           const int *rowk = W.rows + k0 * nj + jj;
-          auto step = [&](R val) {
+          const R *dkp = dkt + kb + k0;
+          auto step = [&](R val, R dkr) {
             R t = val * (R)g;
-            const R dkr = (R)dk;
 #pragma unroll
             for (int l = 0; l <= LP; l++) {
               M[l] += t;
@@ -513,7 +535,6 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
             }
             g *= q;
             q *= qq;
-            dk += 1.0;
           };
           // row-table entry v -> is the column's point in the row; the table
           // read is unconditional (every lane's rowk lies inside the table),
@@ -529,17 +550,21 @@ __device__ __forceinline__ void col_walk(const Task &T, const Layout &lay, const
           auto batch = [&](auto nb) {
             constexpr int NB = decltype(nb)::value;
             R val[NB];
-            bool in[NB];
 #pragma unroll
             for (int u = 0; u < NB; u++) {
-              in[u] = inside(rowk[u * nj]);
-              val[u] = *(GVal *)(gg + gz * plane_bytes + (in[u] ? cb : 0u));
-              gz = (gz + 1 == ngz) ? 0 : gz + 1;
+              val[u] = plane_load<R>(plane, plane_bytes, inside(rowk[u * nj]) ? cb : kPlaneOutside);
+              if (++gz == ngz) {
+                gz = 0;
+                plane = gbase;
+              } else {
+                plane += plane_bytes;
+              }
             }
             rowk += NB * nj;
-            kk += NB;
 #pragma unroll
-            for (int u = 0; u < NB; u++) step(in[u] ? val[u] : R(0));
+            for (int u = 0; u < NB; u++) step(val[u], dkp[u]);
+            dkp += NB;
+            kk += NB;
           };
           while (kk + kColBatch - 1 <= k1) batch(std::integral_constant<int, kColBatch>());
           if (kk + 3 <= k1) batch(std::integral_constant<int, 4>());

@@ -141,7 +141,7 @@ struct WavePlan {
   int o_cxyz, o_u; // doubles: cxyz [np^3]; {cab, alpha} (setup/finish) or tables [3][ext][np]
   int bytes;       // dynamic LDS per workgroup
   // general-cell column walk (integrate), byte offsets in the wave region:
-  // ratio tables (doubles, q_cap), row table of a slab of planes (rows_cap
+  // ratio tables and plane offsets (doubles, q_cap), row table of a slab of planes (rows_cap
   // ints), per row j its footprint (int4, j_cap) and column start (j_cap + 1)
   int o_q, q_cap, o_rows, rows_cap, o_jinfo, o_cstart, j_cap;
 };
