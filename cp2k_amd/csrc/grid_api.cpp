@@ -417,6 +417,8 @@ struct ListInputs {
 
 // Largest LDS tile in doubles; the rest of LDS holds the wave regions.
 constexpr long long kTileDoubles = 14336;
+// waves of a general tile workgroup (lp <= 4; 8 for lp 5..6)
+constexpr int kGenTileWaves = 16;
 
 // Orthorhombic row start, src/grid/ref/grid_ref_collint.h:147 / :49 (the
 // device copy is ortho_start in grid_device.h).
@@ -1144,6 +1146,7 @@ void build_task_list(TaskListImpl *tl, bool orthorhombic, int ntasks, int nlevel
         fits = fits && tasks[s2].lo[d] > -32768 && tasks[s2].hi[d] < 32767;
     if (!fits) continue;
     g.tiled = true;
+    g.max_waves = kGenTileWaves;
     const int nty = (int)((ny + g.by - 1) / g.by);
     struct Seg { int a, b, t; };
     auto segments = [&](const Task &T, int d, int bsz, std::vector<Seg> &out) {
@@ -1650,7 +1653,7 @@ WavePlan general_col_plan(const TaskListImpl *tl, int level, int lp) {
 // level): the largest tile (row stride nx | 1) and wave regions holding the
 // packed cxyz and the dense cijk.
 bool general_tile_plan(const TaskListImpl *tl, int level, int lp, WavePlan &w) {
-  const int cap = level >= 0 ? tl->glev[level].max_waves : 12;
+  const int cap = level >= 0 ? tl->glev[level].max_waves : kGenTileWaves;
   w = WavePlan{};
   w.level = level;
   w.np = lp + 1;
@@ -1662,8 +1665,8 @@ bool general_tile_plan(const TaskListImpl *tl, int level, int lp, WavePlan &w) {
     if ((level < 0 || l == level) && tl->glev[l].tiled)
       win = std::max(win, (long long)(tl->npts_local[3 * l] | 1) * tl->glev[l].by * tl->glev[l].bz);
   if (win <= 0) return false;
-  const int most = std::min(lp <= 4 ? 12 : 8, cap);
-  for (int nw : {12, 10, 8, 6, 4})
+  const int most = std::min(lp <= 4 ? kGenTileWaves : 8, cap);
+  for (int nw : {16, 12, 10, 8, 6, 4})
     if (nw <= most && 8 * win + (long long)nw * w.wave_bytes <= lds_limit()) {
       w.window = (int)win;
       w.nwaves = nw;
@@ -3662,7 +3665,8 @@ void grid_mi355x_autotune(const grid_task_list *task_list, const enum grid_func 
       clear_plans();
       float best = call_ms();
       for (int r : {2, 4, 8})
-        for (int w : {12, 8, 6}) {
+        for (int w : {16, 12, 8, 6}) {
+          if (w > (lv == &tl->glev ? kGenTileWaves : 12)) continue;
           if (r == best_r && w == best_w) continue;
           o.rounds = r;
           o.max_waves = w;

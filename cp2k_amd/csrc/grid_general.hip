@@ -741,7 +741,7 @@ __global__ __launch_bounds__(256) void general_collocate_kernel(const Task *__re
 // P.unit_base + blockIdx.x, the items U.ibeg + U.first + m * U.stride < U.iend
 // of one tile of level U.level; the tile is added to HBM once.
 template <int LPB, typename R>
-__global__ __launch_bounds__(LPB <= 4 ? 768 : 512) void general_tile_kernel(
+__global__ __launch_bounds__(LPB <= 4 ? 1024 : 512) void general_tile_kernel(
     const Task *__restrict__ tasks, const Level *__restrict__ levels,
     const TileUnit *__restrict__ units, const TileItem *__restrict__ items, PointParams P) {
   extern __shared__ double lds[];
@@ -873,7 +873,7 @@ hipError_t launch_general_collocate(const PointParams &P, int lpb, hipStream_t s
 // nunits workgroups from unit P.unit_base; P.wp.window: the largest tile.
 hipError_t launch_general_tiles(const PointParams &P, int lpb, int nunits, hipStream_t s) {
   if (nunits <= 0) return hipSuccess;
-  if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 12 : 8) || P.wp.window <= 0 ||
+  if (P.wp.nwaves < 1 || P.wp.nwaves > (lpb <= 4 ? 16 : 8) || P.wp.window <= 0 ||
       P.wp.np > lpb + 1)
     return hipErrorInvalidValue;
   count_launch(13, P.fp32);
