@@ -796,7 +796,7 @@ __global__ __launch_bounds__(LPB <= 4 ? 1024 : 512) void general_tile_kernel(
 // lp LO..HI: one exact lp keeps the register count (lane sums of lp <= 6)
 // at what that lp needs; LO >= 7 sums on the matrix cores.
 template <int LO, int HI, typename R>
-__global__ __launch_bounds__(256) void general_integrate_kernel(const Task *__restrict__ tasks,
+__global__ __launch_bounds__(256, (HI <= 4 ? 4 : (HI == 5 ? 3 : 2))) void general_integrate_kernel(const Task *__restrict__ tasks,
                                                                 const Level *__restrict__ levels,
                                                                 PointParams P) {
   extern __shared__ double lds[];
