@@ -794,9 +794,11 @@ __global__ __launch_bounds__(LPB <= 4 ? 1024 : 512) void general_tile_kernel(
 // the task's cxyz (packed) for the cxyz -> hab pass.  R = float reads
 // Level::gridf and sums each column's moments in FP32.  The kernel covers
 // lp LO..HI: one exact lp keeps the register count (lane sums of lp <= 6)
-// at what that lp needs; LO >= 7 sums on the matrix cores.
+// at what that lp needs; LO >= 7 sums on the matrix cores.  FP64 asks for 4
+// waves per SIMD up to lp 4 and 3 at lp 5 (a few spilled registers, measured
+// faster); FP32 keeps its registers (there the spills only added traffic).
 template <int LO, int HI, typename R>
-__global__ __launch_bounds__(256, (HI <= 4 ? 4 : (HI == 5 ? 3 : 2))) void general_integrate_kernel(const Task *__restrict__ tasks,
+__global__ __launch_bounds__(256, (sizeof(R) == 8 ? (HI <= 4 ? 4 : (HI == 5 ? 3 : 2)) : 1)) void general_integrate_kernel(const Task *__restrict__ tasks,
                                                                 const Level *__restrict__ levels,
                                                                 PointParams P) {
   extern __shared__ double lds[];
