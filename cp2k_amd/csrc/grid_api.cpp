@@ -2150,6 +2150,9 @@ StagePlan stage_plan(const TaskListImpl *tl) {
   const long long A = std::max(tl->max_nsa, 1), B = std::max(tl->max_nsb, 1);
   const long long CA = std::max(tl->max_ncoa, 1), CB = std::max(tl->max_ncob, 1);
   const long long budget = 2048;  // doubles per wave (16 KiB)
+  // the kernels' index arithmetic (fdiv) holds for dimensions up to 256 and
+  // matrices of fewer than 4096 elements
+  if (std::max(std::max(A, B), std::max(CA, CB)) > 256) return p;
   auto fit = [&](long long fixed, long long per, int &nc, int &wd) {
     const long long n = std::max<long long>(1, std::min(CA, (budget - fixed) / per));
     nc = (int)n;
@@ -2157,6 +2160,8 @@ StagePlan stage_plan(const TaskListImpl *tl) {
   };
   fit(A * B + B * CB, A + B, p.nc_d, p.wd_d);
   fit(B * CB + A * B, CB + B + A, p.nc_c, p.wd_c);
+  const long long ncm = std::max(p.nc_d, p.nc_c);
+  if (A * B >= 4096 || B * CB >= 4096 || std::max(std::max(A, B), CB) * ncm >= 4096) return p;
   const long long wd = std::max(p.wd_d, p.wd_c);
   const long long limit = lds_limit();
   if (8 * wd > limit) return p;

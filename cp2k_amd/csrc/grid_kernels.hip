@@ -853,6 +853,15 @@ __global__ __launch_bounds__(64 * kGroupWaves) void contract_kernel(
 // group computes) and later written to the wave's LDS.  kStageRegs elements
 // per lane and matrix; the host uses the kernels only when every matrix of
 // the list fits.
+// e / n for 0 <= e < 4096 and 1 <= n <= 256 (the staged kernels' operand
+// indices: the host uses them only for matrices within these bounds)
+// through a single-precision reciprocal instead of an integer division:
+// (e + 1/2) / n lies at least 1/512 from an integer, and rounding moves
+// (e + 1/2) rcp(n) by less than 4096 x 2^-22 < 1/1024.
+__device__ __forceinline__ int fdiv(int e, int n) {
+  return (int)(((float)e + 0.5f) * __builtin_amdgcn_rcpf((float)n));
+}
+
 constexpr int kStageRegs = 8;
 
 struct StageRegs {
@@ -868,19 +877,19 @@ __device__ __forceinline__ void dc_load(const Group &g, const double *pab, const
     const int e = lane + 64 * u;
     if (e < g.nsa * g.nsb) {
       if (g.transpose) {
-        const int sb = e / g.nsa, sa = e - sb * g.nsa;
+        const int sb = fdiv(e, g.nsa), sa = e - sb * g.nsa;
         R.p[u] = blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa];
       } else {
-        const int sa = e / g.nsb, sb = e - sa * g.nsb;
+        const int sa = fdiv(e, g.nsb), sb = e - sa * g.nsb;
         R.p[u] = blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
       }
     }
     if (e < g.nsb * g.ncob) {
-      const int sb = e / g.ncob, cb = e - sb * g.ncob;
+      const int sb = fdiv(e, g.ncob), cb = e - sb * g.ncob;
       R.b[u] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
     }
     if (e < g.nsa * nc) {
-      const int sa = e / nc, c = e - sa * nc;
+      const int sa = fdiv(e, nc), c = e - sa * nc;
       R.a[u] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c];
     }
   }
@@ -893,7 +902,7 @@ __device__ __forceinline__ void dc_store(const Group &g, int nc, int lane, const
     const int e = lane + 64 * u;
     if (e < g.nsa * g.nsb) {
       if (g.transpose) {
-        const int sb = e / g.nsa, sa = e - sb * g.nsa;
+        const int sb = fdiv(e, g.nsa), sa = e - sb * g.nsa;
         P[sa * g.nsb + sb] = R.p[u];
       } else {
         P[e] = R.p[u];
@@ -945,15 +954,15 @@ __global__ __launch_bounds__(256) void decontract_staged_kernel(
       const double *blk = pab + g.block_off;
       for (int e = lane; e < nsa * nsb; e += 64) {
         if (g.transpose) {
-          const int sb = e / nsa, sa = e - sb * nsa;
+          const int sb = fdiv(e, nsa), sa = e - sb * nsa;
           P[sa * nsb + sb] = blk[(long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa];
         } else {
-          const int sa = e / nsb, sb = e - sa * nsb;
+          const int sa = fdiv(e, nsb), sb = e - sa * nsb;
           P[e] = blk[(long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb];
         }
       }
       for (int e = lane; e < nsb * ncob; e += 64) {
-        const int sb = e / ncob, cb = e - sb * ncob;
+        const int sb = fdiv(e, ncob), cb = e - sb * ncob;
         Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
       }
     }
@@ -967,20 +976,20 @@ __global__ __launch_bounds__(256) void decontract_staged_kernel(
       const int nc = min(nc_max, ncoa - c0);
       if (!PF || c0 > 0) {
         for (int e = lane; e < nsa * nc; e += 64) {
-          const int sa = e / nc, c = e - sa * nc;
+          const int sa = fdiv(e, nc), c = e - sa * nc;
           Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
         }
       }
       wave_sync();
       for (int e = lane; e < nsb * nc; e += 64) {
-        const int sb = e / nc, c = e - sb * nc;
+        const int sb = fdiv(e, nc), c = e - sb * nc;
         double acc = 0.0;
         for (int sa = 0; sa < nsa; sa++) acc = fma(P[sa * nsb + sb], Sa[sa * nc + c], acc);
         W[e] = acc;
       }
       wave_sync();
       for (int e = lane; e < ncob * nc; e += 64) {
-        const int cb = e / nc, c = e - cb * nc;
+        const int cb = fdiv(e, nc), c = e - cb * nc;
         double acc = 0.0;
         for (int sb = 0; sb < nsb; sb++) acc = fma(Sb[sb * ncob + cb], W[sb * nc + c], acc);
         const long long q = g.cart_off + (long long)cb * ncoa + c0 + c;
@@ -1001,15 +1010,15 @@ __device__ __forceinline__ void ct_load(const Group &g, const double *hab_cart,
   for (int u = 0; u < kStageRegs; u++) {
     const int e = lane + 64 * u;
     if (e < g.ncob * nc) {
-      const int cb = e / nc, c = e - cb * nc;
+      const int cb = fdiv(e, nc), c = e - cb * nc;
       R.p[u] = hab_cart[g.cart_off + (long long)cb * g.ncoa + c];
     }
     if (e < g.nsb * g.ncob) {
-      const int sb = e / g.ncob, cb = e - sb * g.ncob;
+      const int sb = fdiv(e, g.ncob), cb = e - sb * g.ncob;
       R.b[u] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
     }
     if (e < g.nsa * nc) {
-      const int sa = e / nc, c = e - sa * nc;
+      const int sa = fdiv(e, nc), c = e - sa * nc;
       R.a[u] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c];
     }
   }
@@ -1050,7 +1059,7 @@ __global__ __launch_bounds__(256) void contract_staged_kernel(
       }
     } else {
       for (int e = lane; e < nsb * ncob; e += 64) {
-        const int sb = e / ncob, cb = e - sb * ncob;
+        const int sb = fdiv(e, ncob), cb = e - sb * ncob;
         Sb[e] = sphi[g.sphi_b + (long long)sb * g.maxcob + cb];
       }
     }
@@ -1064,24 +1073,24 @@ __global__ __launch_bounds__(256) void contract_staged_kernel(
       const int nc = min(nc_max, ncoa - c0);
       if (!PF || c0 > 0) {
         for (int e = lane; e < ncob * nc; e += 64) {
-          const int cb = e / nc, c = e - cb * nc;
+          const int cb = fdiv(e, nc), c = e - cb * nc;
           H[e] = hab_cart[g.cart_off + (long long)cb * ncoa + c0 + c];
         }
         for (int e = lane; e < nsa * nc; e += 64) {
-          const int sa = e / nc, c = e - sa * nc;
+          const int sa = fdiv(e, nc), c = e - sa * nc;
           Sa[e] = sphi[g.sphi_a + (long long)sa * g.maxcoa + c0 + c];
         }
       }
       wave_sync();
       for (int e = lane; e < nsb * nc; e += 64) {
-        const int sb = e / nc, c = e - sb * nc;
+        const int sb = fdiv(e, nc), c = e - sb * nc;
         double s = 0.0;
         for (int cb = 0; cb < ncob; cb++) s = fma(Sb[sb * ncob + cb], H[cb * nc + c], s);
         W[e] = s;
       }
       wave_sync();
       for (int e = lane; e < nsa * nsb; e += 64) {
-        const int sb = e / nsa, sa = e - sb * nsa;
+        const int sb = fdiv(e, nsa), sa = e - sb * nsa;
         double s = 0.0;
         for (int c = 0; c < nc; c++) s = fma(W[sb * nc + c], Sa[sa * nc + c], s);
         acc[e] += s;
@@ -1089,7 +1098,7 @@ __global__ __launch_bounds__(256) void contract_staged_kernel(
       wave_sync();
     }
     for (int e = lane; e < nsa * nsb; e += 64) {
-      const int sb = e / nsa, sa = e - sb * nsa;
+      const int sb = fdiv(e, nsa), sa = e - sb * nsa;
       const long long q = g.block_off + (g.transpose ? (long long)(g.sb0 + sb) * g.nsgfa + g.sa0 + sa
                                                      : (long long)(g.sa0 + sa) * g.nsgfb + g.sb0 + sb);
       if (q < hab_len) hab[q] = acc[e];
