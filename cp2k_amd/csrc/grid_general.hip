@@ -5,18 +5,22 @@
 // the cube around its sphere) and, in each row (j, k) of the box whose grid
 // indices lie inside the border-masked local grid, the points i of the
 // quadratic bound of src/grid/ref/grid_ref_collint.h:446-464.  One wave
-// takes one task; each lane walks whole rows.  Along a row r(i) = r0 + i h0
-// (h0 = dh[0]), so the Gaussian follows the recurrence
+// takes one task (or one tile item of a task).  Along a line of points the
+// Gaussian follows a recurrence, e.g. along a row r(i) = r0 + i h0
+// (h0 = dh[0]):
 //   g(i + 1) = g(i) q(i),   q(i + 1) = q(i) exp(-2 zetp |h0|^2),
 // restarted from exp() every kRestart points; a point costs two multiplies
 // instead of the reference's three-table product (:358-367).
 //
 // Collocate: cxyz (from the cab -> cxyz pass) -> cijk, the polynomial in the
-// lattice offsets (di, dj, dk) (:697-762); per row ci[il] = sum cijk dj^jl
-// dk^kl, per point a Horner sum in di times the Gaussian; 64 rows x 8 points
-// are staged in LDS and added to the level grid in HBM point-major.  Integrate: per row the moments sum_i V g di^il, folded into
-// lane-private cijk accumulators, one cross-lane reduction per task, cijk ->
-// cxyz, then the shared cxyz -> hab/forces pass.
+// lattice offsets (di, dj, dk) (:697-762); a lane walks whole rows, per row
+// ci[il] = sum cijk dj^jl dk^kl, per point a Horner sum in di times the
+// Gaussian, added into an LDS tile of the level grid (or, without tiles, 64
+// rows x 8 points staged in LDS and added to HBM point-major).  Integrate: a
+// lane walks a column (i, j) along k (the rows tabulated in LDS say which
+// points belong to the task), per column the moments sum_k V g dk^kl folded
+// into lane-private cijk accumulators, one cross-lane reduction per task,
+// cijk -> cxyz, then the shared cxyz -> hab/forces pass.
 //
 // Compiled with -ffp-contract=off; fused multiply-adds are explicit fma().
 
